@@ -1,0 +1,178 @@
+/*
+ * rlc.h — C ABI of the MI355X-native memory-efficient label propagation library
+ * (weighted Misra-Gries / Boyer-Moore sketch LPA, "RAK lowmem").
+ *
+ * Plain C types only: pointers, sizes, PODs. No exceptions cross this boundary; every
+ * entry point returns an rlc_status and leaves a thread-local message for rlc_last_error().
+ *
+ * Reference interfaces replaced (puzzlef/rak-lowmem-communities-cuda):
+ *   - rlc_options           <- RakOptions                              inc/rak.hxx:19-40
+ *   - rlc_run_stats         <- RakResult<K> (iterations/time/...)      inc/rak.hxx:53-97
+ *   - rlc_graph_upload      <- host CSR build + cudaMalloc/cudaMemcpy  inc/rakLowmemCuda.hxx:552-583
+ *   - rlc_run               <- rakLowmemInvokeCuda / rakLowmemStaticCuda<SLOTS>
+ *                                                                     inc/rakLowmemCuda.hxx:538-607, :624-632
+ *   - rlc_modularity        <- modularityBy + communities (quality gate of main.cxx)
+ *                                                                     inc/properties.hxx:177-192, :352-364;
+ *                                                                     main.cxx:44-48, :61, :69
+ *   - rlc_gen_*             <- no counterpart (the reference reads MTX files, inc/mtx.hxx);
+ *                              seeded synthetic inputs for the benchmark configurations.
+ *   - rlc_batch_*           <- no counterpart: many graphs packed in one CSR arena, one launch.
+ *   - rlc_server_*          <- no counterpart: continuous-batching request loop.
+ */
+#ifndef RLC_H
+#define RLC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RLC_ABI_VERSION 1
+#define RLC_MAX_ITER_STATS 64
+#define RLC_EMPTY_LABEL 0xFFFFFFFFu /* reserved: never a valid label */
+
+typedef enum rlc_status {
+  RLC_OK = 0,
+  RLC_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad option value */
+  RLC_ERR_INVALID_GRAPH = 2,    /* non-monotone offsets, edge id >= n, sizes out of range */
+  RLC_ERR_OUT_OF_MEMORY = 3,    /* device or host allocation failed */
+  RLC_ERR_DEVICE = 4,           /* a HIP runtime call failed */
+  RLC_ERR_NO_DEVICE = 5,        /* no HIP device visible */
+  RLC_ERR_CAPACITY = 6,         /* request can never fit the configured memory budget */
+  RLC_ERR_CANCELLED = 7,        /* request cancelled before it ran */
+  RLC_ERR_NOT_READY = 8,        /* request still queued or running */
+  RLC_ERR_BUSY = 9,             /* object in use by another call */
+  RLC_ERR_INTERNAL = 10
+} rlc_status;
+
+/* Execution mode. */
+enum {
+  RLC_MODE_ASYNC = 0, /* in-place label updates (the reference GPU/CPU behaviour); racy, fastest */
+  RLC_MODE_SYNC = 1   /* deterministic: Jacobi label snapshot, changes applied after each iteration */
+};
+
+/* Sketch: rakLowmemStaticCuda<SLOTS>: SLOTS=1 Boyer-Moore, SLOTS=8 Misra-Gries. */
+enum { RLC_SKETCH_BM = 1, RLC_SKETCH_MG8 = 8 };
+
+/* Timing of the iteration loop. */
+enum {
+  RLC_TIMING_GRAPH = 0,        /* all iterations enqueued at once (hipGraph), convergence decided on the
+                                  device, one host synchronisation per run */
+  RLC_TIMING_PER_ITERATION = 1 /* one synchronisation per iteration, HIP events around each move launch
+                                  (fills rlc_run_stats.iter_ms) */
+};
+
+typedef struct rlc_options {
+  int32_t repeat;         /* RakOptions::repeat [1] */
+  double tolerance;       /* RakOptions::tolerance [0.05]: stop when changed/N <= tolerance */
+  int32_t max_iterations; /* RakOptions::maxIterations [20] */
+  int32_t slots;          /* RLC_SKETCH_BM or RLC_SKETCH_MG8 [8] */
+  int32_t mode;           /* RLC_MODE_ASYNC or RLC_MODE_SYNC [ASYNC] */
+  int32_t pickstep;       /* iterations l with l % pickstep == 0 only move to smaller labels and skip the
+                             convergence test; 0 disables [8] */
+  int32_t merge_degree;   /* SYNC: vertices of degree >= merge_degree use the P-part merge-tree sketch,
+                             0 = never (every vertex scans its edges sequentially) [0] */
+  int32_t merge_parts;    /* SYNC: P, 64 or 256 [64] */
+  int32_t wave_degree;    /* ASYNC: degree >= wave_degree -> one wavefront per vertex (0 = library default) */
+  int32_t block_degree;   /* ASYNC: degree >= block_degree -> one workgroup per vertex (0 = library default) */
+  int32_t timing;         /* RLC_TIMING_* [GRAPH] */
+  int32_t reserved[7];    /* must be zero */
+} rlc_options;
+
+typedef struct rlc_run_stats {
+  int32_t iterations;       /* RakResult::iterations */
+  float time_ms;            /* RakResult::time: init + mark + iterations, averaged over repeat */
+  float marking_ms;         /* RakResult::markingTime */
+  float init_ms;            /* RakResult::initializationTime */
+  float memory_gb;          /* RakResult::memory: device working memory excluding the graph */
+  uint32_t changed_last;    /* vertices changed in the last iteration */
+  uint64_t traversed_edges; /* sum over iterations of the degrees of processed (active) vertices */
+  uint64_t algorithmic_bytes; /* sum over iterations of the algorithmic HBM bytes */
+  /* per iteration l < RLC_MAX_ITER_STATS, of the last repeat */
+  uint64_t active[RLC_MAX_ITER_STATS];        /* A_l: processed vertices */
+  uint64_t edges[RLC_MAX_ITER_STATS];         /* E_l: sum of their degrees */
+  uint64_t changed[RLC_MAX_ITER_STATS];       /* C_l: vertices that changed label */
+  uint64_t changed_edges[RLC_MAX_ITER_STATS]; /* X_l: sum of degrees of changed vertices */
+  float iter_ms[RLC_MAX_ITER_STATS];          /* move-launch time (RLC_TIMING_PER_ITERATION only) */
+} rlc_run_stats;
+
+typedef struct rlc_context rlc_context;
+typedef struct rlc_graph rlc_graph;
+
+typedef struct rlc_graph_info {
+  uint64_t n, m;
+  uint64_t max_degree;
+  uint64_t device_bytes; /* graph arrays + per-graph state */
+  int32_t unit_weights;  /* weights were NULL (no weight array stored on the device) */
+  int32_t offsets_bytes; /* 8 */
+} rlc_graph_info;
+
+/* ---- errors -------------------------------------------------------------------------------- */
+const char *rlc_last_error(void);
+const char *rlc_status_string(rlc_status s);
+int32_t rlc_abi_version(void);
+
+/* ---- options ------------------------------------------------------------------------------- */
+/* RakOptions() defaults, plus slots=8, async, pickstep=8 (rakLowmemCuda.hxx:473). */
+void rlc_options_default(rlc_options *o);
+
+/* ---- context ------------------------------------------------------------------------------- */
+rlc_status rlc_context_create(int32_t device, rlc_context **out);
+void rlc_context_destroy(rlc_context *ctx);
+/* free and total device memory (hipMemGetInfo) */
+rlc_status rlc_device_memory(rlc_context *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- graphs -------------------------------------------------------------------------------- */
+/* Upload a host CSR (n vertices, m directed entries, offsets[n+1] with offsets[0]=0,
+ * offsets[n]=m, edges[m] < n, adjacency of each vertex in the order it is scanned).
+ * weights == NULL means every weight is 1.0f (no weight array is stored). Validates the CSR. */
+rlc_status rlc_graph_upload(rlc_context *ctx, uint64_t n, uint64_t m, const uint64_t *offsets,
+                            const uint32_t *edges, const float *weights, rlc_graph **out);
+/* Same, with 32-bit offsets as the reference keeps on the device (rakLowmemCuda.hxx:542). */
+rlc_status rlc_graph_upload32(rlc_context *ctx, uint64_t n, uint64_t m, const uint32_t *offsets,
+                              const uint32_t *edges, const float *weights, rlc_graph **out);
+rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info);
+void rlc_graph_destroy(rlc_graph *g);
+
+/* ---- label propagation --------------------------------------------------------------------- */
+/* Run LPA on an uploaded graph. init_labels (host, n entries, each != RLC_EMPTY_LABEL) may be
+ * NULL: labels start as vertex ids. out_labels (host, n entries) may be NULL: the labels stay
+ * resident on the device (rlc_graph_labels_device, rlc_modularity with labels == NULL). */
+rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, const rlc_options *o,
+                   uint32_t *out_labels, rlc_run_stats *stats);
+/* Device pointer of the resident labels of g (valid until the next run or destroy). */
+const uint32_t *rlc_graph_labels_device(const rlc_graph *g);
+
+/* FP64 modularity on the device, Q = sum_c cin_c/(2M) - (tot_c/(2M))^2 with 2M = sum of all
+ * weights, and the number of distinct labels. labels: host array of n entries, or NULL to use
+ * the resident labels of the last run. */
+rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels, double *modularity,
+                          uint64_t *communities);
+
+/* ---- synthetic inputs (host, seeded, deterministic for any thread count) ------------------- */
+typedef struct rlc_csr {
+  uint64_t n, m;
+  uint64_t *offsets; /* n+1 */
+  uint32_t *edges;   /* m */
+  float *weights;    /* m */
+} rlc_csr;
+
+/* Stochastic block model: n vertices in `blocks` near-equal blocks, expected degree `degree`,
+ * fraction `intra` of edges inside blocks; undirected, no self-loops, sorted adjacency, unit weights. */
+rlc_status rlc_gen_sbm(uint64_t n, uint32_t blocks, double degree, double intra, uint64_t seed, rlc_csr *out);
+/* R-MAT: 2^scale vertices, edge_factor * 2^scale generated edges with quadrant probabilities
+ * a, b, c (d = 1-a-b-c); symmetrised, deduplicated, self-loops dropped, sorted adjacency.
+ * wmin == wmax gives constant weights, else w ~ U[wmin, wmax) hashed per unordered pair. */
+rlc_status rlc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, float wmin,
+                        float wmax, uint64_t seed, rlc_csr *out);
+/* Road-like grid: rows x cols 4-neighbour lattice, each lattice edge dropped with probability
+ * `drop`, plus shortcuts (fraction `shortcut` of the kept lattice edges) to uniform random vertices. */
+rlc_status rlc_gen_grid(uint32_t rows, uint32_t cols, double drop, double shortcut, uint64_t seed, rlc_csr *out);
+void rlc_csr_release(rlc_csr *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RLC_H */

@@ -1,0 +1,181 @@
+"""ORACLE — test infrastructure only.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py import this package, as the
+checker. The library never imports, links or calls it.
+
+  oracle.cpp        C++ restatement of the reference CPU algorithms (liboracle.so)
+  ref_harness.cpp   harness around the reference's own CPU headers (_ref/libref.so), built only
+                    where the reference tree is present
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libref.so")
+
+_P = C.c_void_p
+_u64, _u32, _i32, _i64, _d = C.c_uint64, C.c_uint32, C.c_int, C.c_int64, C.c_double
+
+_ORC = [
+    ("orc_abi_version", _i32, []),
+    ("orc_sketch_choose", None, [_P, _P, _u64, _i32, _i32, _P, _P]),
+    ("orc_lpa_sync", _i32, [_u64, _P, _P, _P, _P, _i32, _d, _i32, _i32, _i64, _i32, _i32, _P, _P]),
+    ("orc_lowmem_omp", _i32, [_u64, _P, _P, _P, _i32, _d, _i32, _i32, _P, _P, _P]),
+    ("orc_rak_omp", _i32, [_u64, _P, _P, _P, _d, _i32, _i32, _P, _P, _P]),
+    ("orc_modularity", _d, [_u64, _P, _P, _P, _P]),
+    ("orc_communities", _u64, [_u64, _P]),
+]
+_REF = [
+    ("ref_lowmem_omp", _i32, [_u64, _P, _P, _P, _i32, _d, _i32, _i32, _P, _P]),
+    ("ref_rak_omp", _i32, [_u64, _P, _P, _P, _d, _i32, _i32, _P, _P]),
+    ("ref_lpa_sync", _i32, [_u64, _P, _P, _P, _P, _i32, _d, _i32, _i32, _i64, _i32, _P, _P]),
+    ("ref_modularity", _d, [_u64, _P, _P, _P, _P]),
+    ("ref_communities", _u64, [_u64, _P, _P, _P, _P]),
+]
+
+_orc = None
+_ref = None
+
+
+def _load(path, sigs):
+    L = C.CDLL(path)
+    for name, res, args in sigs:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError(f"{ORACLE_SO} is not built (make -C oracle)")
+        _orc = _load(ORACLE_SO, _ORC)
+    return _orc
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = _load(REF_SO, _REF)
+    return _ref
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _csr(g):
+    off = np.ascontiguousarray(g.offsets, dtype=np.uint64)
+    edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
+    wei = None if g.weights is None else np.ascontiguousarray(g.weights, dtype=np.float32)
+    return off, edg, wei
+
+
+# ---- restatement -------------------------------------------------------------------------------
+def sketch_choose(labels, weights, slots=8, parts=1):
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    c, x = C.c_uint32(), C.c_float()
+    orc().orc_sketch_choose(_p(lab), _p(w), len(lab), slots, parts, C.byref(c), C.byref(x))
+    return c.value, x.value
+
+
+def lpa_sync(g, slots=8, tolerance=0.05, max_iterations=20, pickstep=8, merge_degree=0, merge_parts=64,
+             init=None, threads=0):
+    """Deterministic synchronous LPA: (labels, iterations, stats[iter, (A, E, C, X)])."""
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    st = np.zeros(4 * max(max_iterations, 1), np.uint64)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint32)
+    it = orc().orc_lpa_sync(n, _p(off), _p(edg), _p(wei), _p(ini), slots, tolerance, max_iterations, pickstep,
+                            merge_degree, merge_parts, threads, _p(lab), _p(st))
+    return lab, it, st[: 4 * it].reshape(it, 4)
+
+
+def lowmem_omp(g, multi=True, tolerance=0.05, max_iterations=20, threads=1):
+    """rakLowmemStaticOmp restatement: (labels, iterations, time_ms, traversed_edges)."""
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    t, e = C.c_double(), C.c_uint64()
+    it = orc().orc_lowmem_omp(n, _p(off), _p(edg), _p(wei), int(multi), tolerance, max_iterations, threads,
+                              _p(lab), C.byref(t), C.byref(e))
+    return lab, it, t.value, e.value
+
+
+def rak_omp(g, tolerance=0.05, max_iterations=20, threads=1):
+    """rakStaticOmp (GVE-LPA) restatement: (labels, iterations, time_ms, traversed_edges)."""
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    t, e = C.c_double(), C.c_uint64()
+    it = orc().orc_rak_omp(n, _p(off), _p(edg), _p(wei), tolerance, max_iterations, threads, _p(lab),
+                           C.byref(t), C.byref(e))
+    return lab, it, t.value, e.value
+
+
+def modularity(g, labels):
+    off, edg, wei = _csr(g)
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    return orc().orc_modularity(len(off) - 1, _p(off), _p(edg), _p(wei), _p(lab))
+
+
+def communities(labels):
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    return int(orc().orc_communities(len(lab), _p(lab)))
+
+
+# ---- the compiled reference (only where oracle/_ref was built) --------------------------------------
+def ref_lowmem_omp(g, multi=True, tolerance=0.05, max_iterations=20, threads=1):
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    t = C.c_float()
+    it = ref().ref_lowmem_omp(n, _p(off), _p(edg), _p(wei), int(multi), tolerance, max_iterations, threads,
+                              _p(lab), C.byref(t))
+    return lab, it, t.value
+
+
+def ref_rak_omp(g, tolerance=0.05, max_iterations=20, threads=1):
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    t = C.c_float()
+    it = ref().ref_rak_omp(n, _p(off), _p(edg), _p(wei), tolerance, max_iterations, threads, _p(lab), C.byref(t))
+    return lab, it, t.value
+
+
+def ref_lpa_sync(g, slots=8, tolerance=0.05, max_iterations=20, pickstep=8, merge_degree=0, merge_parts=64,
+                 init=None):
+    off, edg, wei = _csr(g)
+    n = len(off) - 1
+    lab = np.empty(n, np.uint32)
+    st = np.zeros(4 * max(max_iterations, 1), np.uint64)
+    ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint32)
+    it = ref().ref_lpa_sync(n, _p(off), _p(edg), _p(wei), _p(ini), slots, tolerance, max_iterations, pickstep,
+                            merge_degree, merge_parts, _p(lab), _p(st))
+    return lab, it, st[: 4 * it].reshape(it, 4)
+
+
+def ref_modularity(g, labels):
+    off, edg, wei = _csr(g)
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    return ref().ref_modularity(len(off) - 1, _p(off), _p(edg), _p(wei), _p(lab))
+
+
+def ref_communities(g, labels):
+    off, edg, wei = _csr(g)
+    lab = np.ascontiguousarray(labels, dtype=np.uint32)
+    return int(ref().ref_communities(len(off) - 1, _p(off), _p(edg), _p(wei), _p(lab)))

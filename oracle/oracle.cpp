@@ -1,0 +1,408 @@
+// ORACLE — test infrastructure only. Nothing in the library links, loads or calls this file;
+// only tests/, the smoke check and the CPU-baseline leg of bench.py do, as the checker.
+//
+// A CPU restatement (C++, OpenMP) of the reference's memory-efficient label propagation and of
+// the quality measure its driver prints. Each function cites the reference lines it follows
+// (paths relative to the puzzlef/rak-lowmem-communities-cuda tree).
+//
+//   orc_lowmem_omp      rakLowmemStaticOmp<MULTI, RESCAN=true, SLOTS=8>   inc/rakLowmem.hxx:25-284
+//                       (MULTI=1: Misra-Gries MG8, MULTI=0: Boyer-Moore). With threads=1 the
+//                       result is deterministic and is pinned bit-exact to the compiled reference.
+//   orc_rak_omp         rakStaticOmp (GVE-LPA)                             inc/rak.hxx:111-327
+//   orc_lpa_sync        the deterministic synchronous mode of this library (no reference
+//                       counterpart: the reference updates labels in place). Per vertex it applies
+//                       the reference's own sketch steps (rakLowmem.hxx:63-89, :115-128, :135-172)
+//                       to a label snapshot with a fresh sketch; vertices of degree >= merge_degree
+//                       use the P-part strided scan merged by a fixed binary tree.
+//   orc_modularity      modularityBy / modularityCommunities / edgeWeight  inc/properties.hxx:67-192
+//   orc_communities     communities(x, vcom).size()                        inc/properties.hxx:352-364
+//   orc_sketch_choose   one vertex's choice from an ordered (label, weight) list, for unit tests.
+//
+// Graph input: CSR with n vertices (ids 0..n-1, all present), 64-bit offsets, uint32 targets,
+// float weights (NULL = all 1.0f). Adjacency order is the scan order (ascending neighbour id, as
+// DiGraph keeps it: inc/_bitset.hxx:245-262).
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+constexpr int SLOTS = 8;
+
+struct Csr {
+  uint64_t n;
+  const uint64_t *off;
+  const uint32_t *edg;
+  const float *wei;
+  float w(uint64_t i) const { return wei ? wei[i] : 1.0f; }
+};
+
+struct Sketch {
+  uint32_t c[SLOTS];
+  float w[SLOTS];
+};
+
+// rakLowmemScanCommunityU (rakLowmem.hxx:63-89), after its self-loop test: add w to every slot
+// holding c; if none does, take the highest-index empty (zero-weight) slot; if none is empty,
+// subtract w from every slot, clamped at zero.
+inline void mg_accumulate(Sketch &s, uint32_t c, float w) {
+  for (int i = 0; i < SLOTS; ++i) s.w[i] += s.c[i] == c ? w : 0.0f;
+  bool has = false;
+  for (int i = 0; i < SLOTS; ++i) has |= s.c[i] == c;
+  if (has) return;
+  int f = -1;
+  for (int i = 0; i < SLOTS; ++i)
+    if (s.w[i] == 0.0f) f = i;
+  if (f >= 0) {
+    s.c[f] = c;
+    s.w[f] = w;
+  } else {
+    for (int i = 0; i < SLOTS; ++i) s.w[i] = std::max(s.w[i] - w, 0.0f);
+  }
+}
+
+// rakLowmemChooseCommunityW (rakLowmem.hxx:164-171): first slot of strictly largest weight,
+// starting from (label 0, weight 0).
+inline void mg_argmax(const uint32_t *c, const float *w, uint32_t &cmax, float &wmax) {
+  cmax = 0;
+  wmax = 0.0f;
+  for (int i = 0; i < SLOTS; ++i)
+    if (w[i] > wmax) { wmax = w[i]; cmax = c[i]; }
+}
+
+// rakLowmemScanCommunitiesMajority step (rakLowmem.hxx:121-125)
+inline void bm_step(uint32_t &mc, float &mw, uint32_t c, float w) {
+  if (c == mc) mw += w;
+  else if (mw > w) mw -= w;
+  else { mc = c; mw = w; }
+}
+
+// ---- the deterministic synchronous mode: one vertex's choice --------------------------------
+// Edge source: ordered (label, weight) pairs with self-loops already removed, given as a
+// functor get(i, c, w) over the vertex's adjacency positions i in [0, deg) that returns false for
+// a self-loop (skipped, but it still occupies position i in the strided split).
+template <class Get>
+void choose_mg(uint64_t deg, int parts, Get get, uint32_t &cbest, float &wbest) {
+  if (parts <= 1) {
+    Sketch s;
+    for (int i = 0; i < SLOTS; ++i) { s.c[i] = EMPTY; s.w[i] = 0.0f; }
+    uint32_t c; float w;
+    for (uint64_t i = 0; i < deg; ++i)
+      if (get(i, c, w)) mg_accumulate(s, c, w);
+    // RESCAN=true: recompute slot weights in scan order (rakLowmem.hxx:154-162)
+    float acc[SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint64_t i = 0; i < deg; ++i)
+      if (get(i, c, w))
+        for (int k = 0; k < SLOTS; ++k) acc[k] += s.c[k] == c ? w : 0.0f;
+    mg_argmax(s.c, acc, cbest, wbest);
+    return;
+  }
+  const int P = parts;
+  std::vector<Sketch> sk(P);
+  for (int p = 0; p < P; ++p)
+    for (int i = 0; i < SLOTS; ++i) { sk[p].c[i] = EMPTY; sk[p].w[i] = 0.0f; }
+  uint32_t c; float w;
+  for (uint64_t i = 0; i < deg; ++i)
+    if (get(i, c, w)) mg_accumulate(sk[i % P], c, w);
+  // fixed binary merge tree: part p absorbs part p+k (its non-zero slots, in slot order)
+  for (int k = 1; k < P; k *= 2)
+    for (int p = 0; p + k < P; p += 2 * k)
+      for (int j = 0; j < SLOTS; ++j)
+        if (sk[p + k].w[j] != 0.0f) mg_accumulate(sk[p], sk[p + k].c[j], sk[p + k].w[j]);
+  const Sketch &cand = sk[0];
+  // rescan: per-part partial sums in scan order, then the same binary tree
+  std::vector<float> acc((size_t)P * SLOTS, 0.0f);
+  for (uint64_t i = 0; i < deg; ++i)
+    if (get(i, c, w))
+      for (int k = 0; k < SLOTS; ++k)
+        if (cand.c[k] == c) acc[(i % P) * SLOTS + k] += w;
+  for (int k = 1; k < P; k *= 2)
+    for (int p = 0; p + k < P; p += 2 * k)
+      for (int s = 0; s < SLOTS; ++s) acc[p * SLOTS + s] = acc[p * SLOTS + s] + acc[(p + k) * SLOTS + s];
+  mg_argmax(cand.c, acc.data(), cbest, wbest);
+}
+
+template <class Get>
+void choose_bm(uint64_t deg, int parts, Get get, uint32_t &cbest, float &wbest) {
+  const int P = parts <= 1 ? 1 : parts;
+  std::vector<uint32_t> mc(P, EMPTY);
+  std::vector<float> mw(P, 0.0f);
+  uint32_t c; float w;
+  for (uint64_t i = 0; i < deg; ++i)
+    if (get(i, c, w)) bm_step(mc[i % P], mw[i % P], c, w);
+  // max-weight tree, keeping the lower part on ties
+  for (int k = 1; k < P; k *= 2)
+    for (int p = 0; p + k < P; p += 2 * k)
+      if (mw[p + k] > mw[p]) { mw[p] = mw[p + k]; mc[p] = mc[p + k]; }
+  cbest = mc[0];
+  wbest = mw[0];
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_abi_version() { return 1; }
+
+// One vertex's choice in the synchronous mode, from an ordered list of (label, weight) pairs
+// (self-loops already removed). slots: 8 (MG) or 1 (BM). parts: 1 = sequential, else P.
+void orc_sketch_choose(const uint32_t *labels, const float *weights, uint64_t k, int slots, int parts,
+                       uint32_t *c_out, float *w_out) {
+  auto get = [&](uint64_t i, uint32_t &c, float &w) { c = labels[i]; w = weights[i]; return true; };
+  if (slots == 1) choose_bm(k, parts, get, *c_out, *w_out);
+  else choose_mg(k, parts, get, *c_out, *w_out);
+}
+
+// Deterministic synchronous LPA (this library's RLC_MODE_SYNC). Returns the iteration count.
+// stats (may be NULL): 4 * max_iterations entries, per iteration (active, edges, changed,
+// changed_edges). Any thread count gives the same result.
+int orc_lpa_sync(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *init,
+                 int slots, double tolerance, int max_iterations, int pickstep, int64_t merge_degree,
+                 int merge_parts, int threads, uint32_t *labels, uint64_t *stats) {
+  Csr g{n, off, edg, wei};
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<uint32_t> snap(n), mv(n, EMPTY);
+  std::vector<uint8_t> aff(n, 1);
+  for (uint64_t u = 0; u < n; ++u) labels[u] = init ? init[u] : (uint32_t)u;
+  int l = 0;
+  while (l < max_iterations) {
+    const bool pickless = pickstep > 0 && l % pickstep == 0;
+    std::copy(labels, labels + n, snap.begin());
+    uint64_t A = 0, E = 0;
+#pragma omp parallel for schedule(dynamic, 2048) reduction(+ : A, E)
+    for (int64_t ui = 0; ui < (int64_t)n; ++ui) {
+      const uint64_t u = (uint64_t)ui;
+      mv[u] = EMPTY;
+      if (!aff[u]) continue;
+      const uint64_t b = g.off[u], deg = g.off[u + 1] - b;
+      ++A;
+      E += deg;
+      const uint32_t d = snap[u];
+      auto get = [&](uint64_t i, uint32_t &c, float &w) {
+        uint32_t v = g.edg[b + i];
+        if (v == (uint32_t)u) return false;
+        c = snap[v];
+        w = g.w(b + i);
+        return true;
+      };
+      const int P = (merge_degree > 0 && (int64_t)deg >= merge_degree) ? merge_parts : 1;
+      uint32_t c;
+      float w;
+      if (slots == 1) choose_bm(deg, P, get, c, w);
+      else choose_mg(deg, P, get, c, w);
+      if (w > 0 && c != d && !(pickless && c > d)) mv[u] = c;
+    }
+    uint64_t C = 0, X = 0;
+    std::fill(aff.begin(), aff.end(), 0);
+    for (uint64_t u = 0; u < n; ++u) {
+      if (mv[u] == EMPTY) continue;
+      labels[u] = mv[u];
+      ++C;
+      X += g.off[u + 1] - g.off[u];
+      for (uint64_t i = g.off[u]; i < g.off[u + 1]; ++i)
+        if (g.edg[i] != (uint32_t)u) aff[g.edg[i]] = 1;
+    }
+    if (stats) {
+      stats[4 * l + 0] = A;
+      stats[4 * l + 1] = E;
+      stats[4 * l + 2] = C;
+      stats[4 * l + 3] = X;
+    }
+    ++l;
+    if (!pickless && double(C) / double(n) <= tolerance) break;
+  }
+  return l;
+}
+
+// rakLowmemStaticOmp<MULTI, RESCAN=true, SLOTS=8> (rakLowmem.hxx:189-284): labels updated in
+// place in vertex order; each thread's sketch labels persist across vertices and iterations
+// (zero-filled once, rakLowmem.hxx:29; only weights are cleared, :200). With threads=1 the
+// result is deterministic. Returns iterations; *time_ms = init + mark + iterations (the
+// reference's measureDuration span, rakLowmem.hxx:251-261); *traversed = sum of processed degrees.
+int orc_lowmem_omp(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, int multi,
+                   double tolerance, int max_iterations, int threads, uint32_t *labels, double *time_ms,
+                   uint64_t *traversed) {
+  Csr g{n, off, edg, wei};
+  if (threads > 0) omp_set_num_threads(threads);
+  const int T = omp_get_max_threads();
+  std::vector<Sketch> sk(T);
+  for (auto &s : sk) for (int i = 0; i < SLOTS; ++i) { s.c[i] = 0; s.w[i] = 0.0f; }
+  std::vector<uint8_t> aff(n);
+  uint64_t trav = 0;
+  double t0 = now_ms();
+  // rakInitializeOmpW (rak.hxx:145-153), fillValueOmpU (rakLowmem.hxx:281)
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; ++u) labels[u] = (uint32_t)u;
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; ++u) aff[u] = 1;
+  int l = 0;
+  for (l = 0; l < max_iterations;) {
+    uint64_t a = 0, e = 0;
+#pragma omp parallel for schedule(dynamic, 2048) reduction(+ : a, e)
+    for (int64_t ui = 0; ui < (int64_t)n; ++ui) {
+      const uint32_t u = (uint32_t)ui;
+      if (!aff[u]) continue;
+      Sketch &s = sk[omp_get_thread_num()];
+      const uint64_t b = g.off[u], end = g.off[u + 1];
+      e += end - b;
+      const uint32_t d = labels[u];
+      uint32_t c;
+      float w;
+      if (multi) {
+        for (int i = 0; i < SLOTS; ++i) s.w[i] = 0.0f;  // rakLowmemClearScanW
+        for (uint64_t i = b; i < end; ++i) {
+          uint32_t v = g.edg[i];
+          if (v == u) continue;
+          mg_accumulate(s, labels[v], g.w(i));
+        }
+        for (int i = 0; i < SLOTS; ++i) s.w[i] = 0.0f;  // rescan
+        for (uint64_t i = b; i < end; ++i) {
+          uint32_t v = g.edg[i];
+          if (v == u) continue;
+          uint32_t cv = labels[v];
+          float wv = g.w(i);
+          for (int k = 0; k < SLOTS; ++k) s.w[k] += s.c[k] == cv ? wv : 0.0f;
+        }
+        mg_argmax(s.c, s.w, c, w);
+      } else {
+        c = 0;  // rakLowmemScanCommunitiesMajority: mc = K(), mw = V()
+        w = 0.0f;
+        for (uint64_t i = b; i < end; ++i) {
+          uint32_t v = g.edg[i];
+          if (v == u) continue;
+          bm_step(c, w, labels[v], g.w(i));
+        }
+      }
+      if (w > 0 && c != d) {
+        labels[u] = c;
+        ++a;
+        for (uint64_t i = b; i < end; ++i) aff[g.edg[i]] = 1;
+      }
+      aff[u] = 0;
+    }
+    trav += e;
+    ++l;
+    if (double(a) / double(n) <= tolerance) break;
+  }
+  if (time_ms) *time_ms = now_ms() - t0;
+  if (traversed) *traversed = trav;
+  return l;
+}
+
+// rakStaticOmp (GVE-LPA, rak.hxx:239-327): exact per-thread dense accumulator vcout[S] (double)
+// with a touched-label list; the first label of strictly largest total in first-touch order wins;
+// label 0 is never adopted (rak.hxx:252).
+int orc_rak_omp(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, double tolerance,
+                int max_iterations, int threads, uint32_t *labels, double *time_ms, uint64_t *traversed) {
+  Csr g{n, off, edg, wei};
+  if (threads > 0) omp_set_num_threads(threads);
+  const int T = omp_get_max_threads();
+  std::vector<std::vector<uint32_t>> vcs(T);
+  std::vector<std::vector<double>> vcout(T, std::vector<double>(n));
+  std::vector<uint8_t> aff(n);
+  uint64_t trav = 0;
+  double t0 = now_ms();
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; ++u) labels[u] = (uint32_t)u;
+#pragma omp parallel for schedule(static)
+  for (int64_t u = 0; u < (int64_t)n; ++u) aff[u] = 1;
+  int l = 0;
+  for (l = 0; l < max_iterations;) {
+    uint64_t a = 0, e = 0;
+#pragma omp parallel for schedule(dynamic, 2048) reduction(+ : a, e)
+    for (int64_t ui = 0; ui < (int64_t)n; ++ui) {
+      const uint32_t u = (uint32_t)ui;
+      if (!aff[u]) continue;
+      const int t = omp_get_thread_num();
+      auto &cs = vcs[t];
+      auto &co = vcout[t];
+      const uint64_t b = g.off[u], end = g.off[u + 1];
+      e += end - b;
+      const uint32_t d = labels[u];
+      for (uint32_t c : cs) co[c] = 0;  // rakClearScanW
+      cs.clear();
+      for (uint64_t i = b; i < end; ++i) {  // rakScanCommunityU
+        uint32_t v = g.edg[i];
+        if (v == u) continue;
+        uint32_t c = labels[v];
+        if (!co[c]) cs.push_back(c);
+        co[c] += g.w(i);
+      }
+      uint32_t cmax = 0;  // rakChooseCommunity
+      double wmax = 0;
+      for (uint32_t c : cs)
+        if (co[c] > wmax) { cmax = c; wmax = co[c]; }
+      if (cmax && cmax != d) {
+        labels[u] = cmax;
+        ++a;
+        for (uint64_t i = b; i < end; ++i) aff[g.edg[i]] = 1;
+      }
+      aff[u] = 0;
+    }
+    trav += e;
+    ++l;
+    if (double(a) / double(n) <= tolerance) break;
+  }
+  if (time_ms) *time_ms = now_ms() - t0;
+  if (traversed) *traversed = trav;
+  return l;
+}
+
+// modularityBy(x, fc, M, R=1) with M = edgeWeight(x)/2 (properties.hxx:67-192, main.cxx:61):
+// cin/ctot indexed by label, summed serially in vertex and adjacency order. Labels >= n are
+// first mapped to dense ids in order of first appearance (the reference indexes by label and
+// needs labels < span).
+double orc_modularity(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *lab) {
+  Csr g{n, off, edg, wei};
+  std::vector<uint32_t> c(n);
+  bool dense = true;
+  for (uint64_t u = 0; u < n; ++u) dense &= lab[u] < n;
+  if (dense) {
+    for (uint64_t u = 0; u < n; ++u) c[u] = lab[u];
+  } else {
+    std::unordered_map<uint32_t, uint32_t> ids;
+    for (uint64_t u = 0; u < n; ++u) {
+      auto it = ids.emplace(lab[u], (uint32_t)ids.size()).first;
+      c[u] = it->second;
+    }
+  }
+  double W = 0;
+  for (uint64_t u = 0; u < n; ++u) {
+    double a = 0;
+    for (uint64_t i = g.off[u]; i < g.off[u + 1]; ++i) a += g.w(i);
+    W += a;
+  }
+  const double M = W / 2;
+  std::vector<double> cin(n), ctot(n);
+  for (uint64_t u = 0; u < n; ++u) {
+    const uint32_t cu = c[u];
+    for (uint64_t i = g.off[u]; i < g.off[u + 1]; ++i) {
+      const uint32_t cv = c[g.edg[i]];
+      if (cu == cv) cin[cu] += g.w(i);
+      ctot[cu] += g.w(i);
+    }
+  }
+  double q = 0;
+  for (uint64_t k = 0; k < n; ++k) q += cin[k] / (2 * M) - std::pow(ctot[k] / (2 * M), 2);
+  return q;
+}
+
+// communities(x, vcom).size() (properties.hxx:352-364): number of distinct labels.
+uint64_t orc_communities(uint64_t n, const uint32_t *lab) {
+  std::vector<uint32_t> v(lab, lab + n);
+  std::sort(v.begin(), v.end());
+  return (uint64_t)(std::unique(v.begin(), v.end()) - v.begin());
+}
+
+}  // extern "C"

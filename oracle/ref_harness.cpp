@@ -1,0 +1,188 @@
+// ORACLE — test infrastructure only (builds oracle/_ref/libref.so; never part of the library).
+//
+// A thin C harness around the reference's own CPU code, compiled from the reference headers
+// where they lie (include path given by the Makefile). Nothing of the reference is copied here:
+// this file only converts a CSR into the reference's DiGraphCsr (inc/Graph.hxx:383-672) and
+// calls the reference functions, so that the oracle restatement can be checked against them.
+//
+//   ref_lowmem_omp   rakLowmemStaticOmp<MULTI, true, 8>          inc/rakLowmem.hxx:277-284
+//   ref_rak_omp      rakStaticOmp                                inc/rak.hxx:320-327
+//   ref_lpa_sync     the library's synchronous mode, driven through the reference's per-vertex
+//                    functions rakLowmemClearScanW / rakLowmemScanCommunitiesW /
+//                    rakLowmemScanCommunityU / rakLowmemChooseCommunityW<false, true> /
+//                    rakLowmemScanCommunitiesMajority (inc/rakLowmem.hxx:63-172) on a Jacobi label
+//                    snapshot with a fresh sketch per vertex (labels = 0xFFFFFFFF, weights = 0)
+//   ref_modularity   modularityBy(x, fc, edgeWeightOmp(x)/2, 1)  inc/properties.hxx:177-192, main.cxx:44-48
+//   ref_communities  communities(x, vcom).size()                 inc/properties.hxx:352-364
+#ifndef BUILD
+#define BUILD 0
+#endif
+#ifndef OPENMP
+#define OPENMP 1
+#endif
+#include <omp.h>
+
+#include <array>
+#include <cstdint>
+#include <vector>
+
+#include "_main.hxx"
+#include "Graph.hxx"
+#include "properties.hxx"
+#include "rak.hxx"
+#include "rakLowmem.hxx"
+
+using K = uint32_t;
+using V = float;
+using G = DiGraphCsr<K, None, V, size_t>;
+
+static G make_graph(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei) {
+  uint64_t m = off[n];
+  G x(n, m);
+  for (uint64_t u = 0; u <= n; ++u) x.offsets[u] = off[u];
+  for (uint64_t u = 0; u < n; ++u) x.degrees[u] = (K)(off[u + 1] - off[u]);
+  for (uint64_t i = 0; i < m; ++i) {
+    x.edgeKeys[i] = edg[i];
+    x.edgeValues[i] = wei ? wei[i] : 1.0f;
+  }
+  return x;
+}
+
+extern "C" {
+
+int ref_lowmem_omp(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, int multi, double tol,
+                   int maxit, int threads, uint32_t *labels, float *time_ms) {
+  omp_set_num_threads(threads);
+  G x = make_graph(n, off, edg, wei);
+  RakOptions o(1, tol, maxit);
+  auto a = multi ? rakLowmemStaticOmp<true, true, 8>(x, o) : rakLowmemStaticOmp<false, true, 8>(x, o);
+  for (uint64_t u = 0; u < n; ++u) labels[u] = a.membership[u];
+  if (time_ms) *time_ms = a.time;
+  return a.iterations;
+}
+
+int ref_rak_omp(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, double tol, int maxit,
+                int threads, uint32_t *labels, float *time_ms) {
+  omp_set_num_threads(threads);
+  G x = make_graph(n, off, edg, wei);
+  RakOptions o(1, tol, maxit);
+  auto a = rakStaticOmp(x, o);
+  for (uint64_t u = 0; u < n; ++u) labels[u] = a.membership[u];
+  if (time_ms) *time_ms = a.time;
+  return a.iterations;
+}
+
+int ref_lpa_sync(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *init,
+                 int slots, double tol, int maxit, int pickstep, int64_t merge_degree, int merge_parts,
+                 uint32_t *labels, uint64_t *stats) {
+  const K EMPTY = 0xFFFFFFFFu;
+  G x = make_graph(n, off, edg, wei);
+  vector<K> vcom(n), snap(n), mv(n);
+  vector<char> aff(n, 1);
+  for (uint64_t u = 0; u < n; ++u) vcom[u] = init ? init[u] : (K)u;
+  int l = 0;
+  while (l < maxit) {
+    const bool pickless = pickstep > 0 && l % pickstep == 0;
+    snap = vcom;
+    uint64_t A = 0, E = 0, C = 0, X = 0;
+    for (K u = 0; u < (K)n; ++u) {
+      mv[u] = EMPTY;
+      if (!aff[u]) continue;
+      ++A;
+      E += x.degree(u);
+      K d = snap[u];
+      K c;
+      V w;
+      const uint64_t deg = x.degree(u);
+      const int P = (merge_degree > 0 && (int64_t)deg >= merge_degree) ? merge_parts : 1;
+      if (slots == 1) {
+        if (P == 1) {
+          auto cw = rakLowmemScanCommunitiesMajority<false>(x, u, snap);
+          c = cw.first;
+          w = cw.second;
+        } else {
+          // per-part majority over the strided split, then the max-weight tree
+          vector<K> mc(P, 0);
+          vector<V> mw(P, 0);
+          uint64_t i = 0;
+          x.forEachEdge(u, [&](auto v, auto wv) {
+            int p = (int)(i++ % P);
+            if (u == v) return;
+            K cv = snap[v];
+            if (cv == mc[p]) mw[p] += wv;
+            else if (mw[p] > wv) mw[p] -= wv;
+            else { mc[p] = cv; mw[p] = wv; }
+          });
+          for (int k = 1; k < P; k *= 2)
+            for (int p = 0; p + k < P; p += 2 * k)
+              if (mw[p + k] > mw[p]) { mw[p] = mw[p + k]; mc[p] = mc[p + k]; }
+          c = mc[0];
+          w = mw[0];
+        }
+      } else {
+        if (P == 1) {
+          array<K, 8> mcs;
+          array<V, 8> mws;
+          mcs.fill(EMPTY);
+          rakLowmemClearScanW(mws);
+          rakLowmemScanCommunitiesW<false>(mcs, mws, x, u, snap);
+          auto cw = rakLowmemChooseCommunityW<false, true>(mws, x, u, mcs, snap);
+          c = cw.first;
+          w = cw.second;
+        } else {
+          // P strided parts through the reference's accumulate step, merged by the binary tree
+          // (each non-zero slot of part p+k accumulated into part p), then the reference rescan.
+          vector<array<K, 8>> pc(P);
+          vector<array<V, 8>> pw(P);
+          for (int p = 0; p < P; ++p) { pc[p].fill(EMPTY); rakLowmemClearScanW(pw[p]); }
+          uint64_t i = 0;
+          x.forEachEdge(u, [&](auto v, auto wv) {
+            int p = (int)(i++ % P);
+            rakLowmemScanCommunityU<false>(pc[p], pw[p], u, (K)v, (V)wv, snap);
+          });
+          vector<K> one(1);
+          for (int k = 1; k < P; k *= 2)
+            for (int p = 0; p + k < P; p += 2 * k)
+              for (int j = 0; j < 8; ++j) {
+                if (!pw[p + k][j]) continue;
+                one[0] = pc[p + k][j];  // label fed through vcom[v] with v=0, u=1
+                rakLowmemScanCommunityU<false>(pc[p], pw[p], (K)1, (K)0, pw[p + k][j], one);
+              }
+          auto cw = rakLowmemChooseCommunityW<false, true>(pw[0], x, u, pc[0], snap);
+          c = cw.first;
+          w = cw.second;
+        }
+      }
+      if (w > 0 && c != d && !(pickless && c > d)) mv[u] = c;
+    }
+    std::fill(aff.begin(), aff.end(), 0);
+    for (K u = 0; u < (K)n; ++u) {
+      if (mv[u] == EMPTY) continue;
+      vcom[u] = mv[u];
+      ++C;
+      X += x.degree(u);
+      x.forEachEdgeKey(u, [&](auto v) { if (v != u) aff[v] = 1; });
+    }
+    if (stats) { stats[4 * l] = A; stats[4 * l + 1] = E; stats[4 * l + 2] = C; stats[4 * l + 3] = X; }
+    ++l;
+    if (!pickless && double(C) / n <= tol) break;
+  }
+  for (uint64_t u = 0; u < n; ++u) labels[u] = vcom[u];
+  return l;
+}
+
+double ref_modularity(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *labels) {
+  omp_set_num_threads(1);
+  G x = make_graph(n, off, edg, wei);
+  double M = edgeWeightOmp(x) / 2;
+  auto fc = [&](auto u) { return labels[u]; };
+  return modularityBy(x, fc, M, 1.0);
+}
+
+uint64_t ref_communities(uint64_t n, const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *labels) {
+  G x = make_graph(n, off, edg, wei);
+  vector<K> vcom(labels, labels + n);
+  return communities(x, vcom).size();
+}
+
+}  // extern "C"

@@ -1,0 +1,502 @@
+// C ABI implementation: contexts, resident graphs, the label-propagation run loop, modularity.
+//
+// Replaces the host side of rakLowmemInvokeCuda (inc/rakLowmemCuda.hxx:538-607): instead of
+// building the CSR, allocating, uploading and freeing on every call, a graph is uploaded once
+// (rlc_graph_upload) and run many times; the iteration loop (rakLowmemLoopCuU, :469-490) is
+// enqueued as a captured hipGraph whose convergence test runs on the device, so a run costs
+// one host synchronisation instead of one per iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "lpa.hpp"
+
+using namespace rlc;
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(RLC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct rlc_context {
+  int device;
+};
+
+namespace {
+
+constexpr uint64_t kNoLimit = ~0ull;
+constexpr int kDefaultWaveDegree = 32;
+constexpr int kDefaultBlockDegree = 2048;
+
+struct ExecKey {
+  bool valid = false;
+  int sync, mg, pickstep, max_iterations;
+  double tolerance;
+  uint64_t t1, t2;
+  bool operator==(const ExecKey &o) const {
+    return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
+           max_iterations == o.max_iterations && tolerance == o.tolerance && t1 == o.t1 && t2 == o.t2;
+  }
+};
+
+template <class T>
+rlc_status dev_alloc(T **p, uint64_t count) {
+  size_t bytes = std::max<size_t>((size_t)count * sizeof(T), 256);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? RLC_ERR_OUT_OF_MEMORY : RLC_ERR_DEVICE,
+                std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
+  return RLC_OK;
+}
+
+}  // namespace
+
+struct rlc_graph {
+  rlc_context *ctx = nullptr;
+  uint64_t n = 0, m = 0, max_degree = 0;
+  bool unit = false;
+  std::vector<uint64_t> h_off;
+  uint64_t *d_off = nullptr;
+  uint32_t *d_edg = nullptr;
+  float *d_wei = nullptr;
+  // degree classes
+  uint32_t *d_list = nullptr;
+  uint64_t list_t1 = 0, list_t2 = 0, c1 = 0, c2 = 0;
+  bool list_valid = false;
+  // state
+  uint32_t *d_lab = nullptr;
+  uint8_t *d_aff = nullptr;
+  uint64_t *d_moves = nullptr;
+  DevCounters *d_ctr = nullptr;
+  uint32_t *d_init = nullptr;
+  double *d_ctot = nullptr;
+  uint8_t *d_seen = nullptr;
+  double *d_modout = nullptr;
+  bool labels_dense = true;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t evm[2] = {nullptr, nullptr};
+  hipGraph_t hgraph = nullptr;
+  hipGraphExec_t hexec = nullptr;
+  ExecKey key;
+  std::atomic<int> busy{0};
+
+  DevGraph dev() const { return DevGraph{d_off, d_edg, unit ? nullptr : d_wei, n, m}; }
+  DevState state() const { return DevState{d_lab, d_aff, d_moves, d_ctr}; }
+};
+
+namespace {
+
+struct BusyGuard {
+  std::atomic<int> &b;
+  bool ok;
+  explicit BusyGuard(std::atomic<int> &x) : b(x) {
+    int z = 0;
+    ok = b.compare_exchange_strong(z, 1);
+  }
+  ~BusyGuard() { if (ok) b.store(0); }
+};
+
+void free_graph(rlc_graph *g) {
+  if (!g) return;
+  if (g->ctx) (void)hipSetDevice(g->ctx->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->hexec) (void)hipGraphExecDestroy(g->hexec);
+  if (g->hgraph) (void)hipGraphDestroy(g->hgraph);
+  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_list, g->d_lab, g->d_aff, g->d_moves,
+                  g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto &e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : g->evm)
+    if (e) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+rlc_status ensure_lists(rlc_graph *g, uint64_t t1, uint64_t t2) {
+  if (g->list_valid && g->list_t1 == t1 && g->list_t2 == t2) return RLC_OK;
+  std::vector<uint32_t> list;
+  uint64_t c1, c2;
+  build_class_lists(g->n, g->h_off.data(), t1, t2, list, c1, c2);
+  if (!g->d_list) {
+    rlc_status st = dev_alloc(&g->d_list, g->n);
+    if (st != RLC_OK) return st;
+  }
+  HIP_TRY(hipMemcpyAsync(g->d_list, list.data(), g->n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->list_t1 = t1;
+  g->list_t2 = t2;
+  g->c1 = c1;
+  g->c2 = c2;
+  g->list_valid = true;
+  g->key.valid = false;  // captured launches embed the class sizes
+  return RLC_OK;
+}
+
+rlc_status check_options(const rlc_options *o) {
+  if (!o) return fail(RLC_ERR_INVALID_ARGUMENT, "options is null");
+  if (o->repeat < 1) return fail(RLC_ERR_INVALID_ARGUMENT, "options.repeat must be >= 1");
+  if (o->max_iterations < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.max_iterations must be >= 0");
+  if (!(o->tolerance >= 0)) return fail(RLC_ERR_INVALID_ARGUMENT, "options.tolerance must be >= 0");
+  if (o->slots != RLC_SKETCH_BM && o->slots != RLC_SKETCH_MG8)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.slots must be 1 (Boyer-Moore) or 8 (Misra-Gries)");
+  if (o->mode != RLC_MODE_ASYNC && o->mode != RLC_MODE_SYNC)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.mode must be RLC_MODE_ASYNC or RLC_MODE_SYNC");
+  if (o->pickstep < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.pickstep must be >= 0");
+  if (o->merge_degree < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_degree must be >= 0");
+  if (o->mode == RLC_MODE_SYNC && o->merge_degree > 0 && o->merge_parts != 64 && o->merge_parts != 256)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts must be 64 or 256");
+  if (o->wave_degree < 0 || o->block_degree < 0)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.wave_degree / block_degree must be >= 0");
+  if (o->timing != RLC_TIMING_GRAPH && o->timing != RLC_TIMING_PER_ITERATION)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.timing must be RLC_TIMING_GRAPH or RLC_TIMING_PER_ITERATION");
+  for (int r : o->reserved)
+    if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.reserved must be zero");
+  return RLC_OK;
+}
+
+// degree thresholds (t1, t2) of the lane / wave / workgroup classes
+void class_thresholds(const rlc_options *o, uint64_t &t1, uint64_t &t2) {
+  if (o->mode == RLC_MODE_SYNC) {
+    if (o->merge_degree <= 0) { t1 = t2 = kNoLimit; return; }
+    t1 = (uint64_t)o->merge_degree;
+    t2 = o->merge_parts == 64 ? kNoLimit : t1;
+    return;
+  }
+  t1 = (uint64_t)(o->wave_degree > 0 ? o->wave_degree : kDefaultWaveDegree);
+  t2 = (uint64_t)(o->block_degree > 0 ? o->block_degree : kDefaultBlockDegree);
+  if (t2 < t1) t2 = t1;
+}
+
+rlc_status check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return RLC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rlc_last_error(void) { return last_error_cstr(); }
+
+const char *rlc_status_string(rlc_status s) {
+  switch (s) {
+    case RLC_OK: return "ok";
+    case RLC_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case RLC_ERR_INVALID_GRAPH: return "invalid graph";
+    case RLC_ERR_OUT_OF_MEMORY: return "out of memory";
+    case RLC_ERR_DEVICE: return "device error";
+    case RLC_ERR_NO_DEVICE: return "no device";
+    case RLC_ERR_CAPACITY: return "exceeds capacity";
+    case RLC_ERR_CANCELLED: return "cancelled";
+    case RLC_ERR_NOT_READY: return "not ready";
+    case RLC_ERR_BUSY: return "busy";
+    case RLC_ERR_INTERNAL: return "internal error";
+  }
+  return "unknown status";
+}
+
+int32_t rlc_abi_version(void) { return RLC_ABI_VERSION; }
+
+void rlc_options_default(rlc_options *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->repeat = 1;
+  o->tolerance = 0.05;
+  o->max_iterations = 20;
+  o->slots = RLC_SKETCH_MG8;
+  o->mode = RLC_MODE_ASYNC;
+  o->pickstep = 8;
+  o->merge_degree = 0;
+  o->merge_parts = 64;
+  o->timing = RLC_TIMING_GRAPH;
+}
+
+rlc_status rlc_context_create(int32_t device, rlc_context **out) {
+  clear_error();
+  if (!out) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_context_create: out is null");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) return fail(RLC_ERR_NO_DEVICE, "no HIP device visible");
+  if (device < 0 || device >= count) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_context_create: bad device index");
+  HIP_TRY(hipSetDevice(device));
+  *out = new rlc_context{device};
+  return RLC_OK;
+}
+
+void rlc_context_destroy(rlc_context *ctx) { delete ctx; }
+
+rlc_status rlc_device_memory(rlc_context *ctx, uint64_t *free_bytes, uint64_t *total_bytes) {
+  clear_error();
+  if (!ctx || !free_bytes || !total_bytes) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_device_memory: null argument");
+  HIP_TRY(hipSetDevice(ctx->device));
+  size_t f = 0, t = 0;
+  HIP_TRY(hipMemGetInfo(&f, &t));
+  *free_bytes = f;
+  *total_bytes = t;
+  return RLC_OK;
+}
+
+static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::vector<uint64_t> &&off,
+                                const uint32_t *edges, const float *weights, rlc_graph **out) {
+  uint64_t md = 0;
+  rlc_status st = validate_csr(n, m, off.data(), edges, weights, &md);
+  if (st != RLC_OK) return st;
+  HIP_TRY(hipSetDevice(ctx->device));
+  rlc_graph *g = new rlc_graph();
+  g->ctx = ctx;
+  g->n = n;
+  g->m = m;
+  g->max_degree = md;
+  g->unit = weights == nullptr;
+  g->h_off = std::move(off);
+  auto bail = [&](rlc_status s) { free_graph(g); return s; };
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "hipStreamCreate failed"));
+  for (auto &e : g->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
+  for (auto &e : g->evm)
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
+  if ((st = dev_alloc(&g->d_off, n + 1)) != RLC_OK) return bail(st);
+  if ((st = dev_alloc(&g->d_edg, m)) != RLC_OK) return bail(st);
+  if (!g->unit && (st = dev_alloc(&g->d_wei, m)) != RLC_OK) return bail(st);
+  if ((st = dev_alloc(&g->d_lab, n)) != RLC_OK) return bail(st);
+  if ((st = dev_alloc(&g->d_aff, n)) != RLC_OK) return bail(st);
+  if ((st = dev_alloc(&g->d_ctr, 1)) != RLC_OK) return bail(st);
+  auto cp = [&](void *dst, const void *src, size_t bytes) {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->stream) : hipSuccess;
+  };
+  if (cp(g->d_off, g->h_off.data(), (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+      cp(g->d_edg, edges, m * sizeof(uint32_t)) != hipSuccess ||
+      (!g->unit && cp(g->d_wei, weights, m * sizeof(float)) != hipSuccess) ||
+      hipStreamSynchronize(g->stream) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "graph upload copy failed"));
+  *out = g;
+  return RLC_OK;
+}
+
+rlc_status rlc_graph_upload(rlc_context *ctx, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
+                            const float *weights, rlc_graph **out) {
+  clear_error();
+  if (!ctx || !out || !offsets) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_upload: null argument");
+  *out = nullptr;
+  if (n >= 0xFFFFFFFFull) return fail(RLC_ERR_INVALID_GRAPH, "graph: n must be below 2^32-1");
+  std::vector<uint64_t> off(offsets, offsets + n + 1);
+  return upload_common(ctx, n, m, std::move(off), edges, weights, out);
+}
+
+rlc_status rlc_graph_upload32(rlc_context *ctx, uint64_t n, uint64_t m, const uint32_t *offsets, const uint32_t *edges,
+                              const float *weights, rlc_graph **out) {
+  clear_error();
+  if (!ctx || !out || !offsets) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_upload32: null argument");
+  *out = nullptr;
+  if (n >= 0xFFFFFFFFull) return fail(RLC_ERR_INVALID_GRAPH, "graph: n must be below 2^32-1");
+  std::vector<uint64_t> off(offsets, offsets + n + 1);
+  return upload_common(ctx, n, m, std::move(off), edges, weights, out);
+}
+
+rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info) {
+  clear_error();
+  if (!g || !info) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_info_get: null argument");
+  info->n = g->n;
+  info->m = g->m;
+  info->max_degree = g->max_degree;
+  info->unit_weights = g->unit ? 1 : 0;
+  info->offsets_bytes = 8;
+  info->device_bytes = (g->n + 1) * 8 + g->m * 4 + (g->unit ? 0 : g->m * 4) + g->n * (4 + 1 + 4) +
+                       (g->d_moves ? g->n * 8 : 0) + sizeof(DevCounters);
+  return RLC_OK;
+}
+
+void rlc_graph_destroy(rlc_graph *g) { free_graph(g); }
+
+const uint32_t *rlc_graph_labels_device(const rlc_graph *g) { return g ? g->d_lab : nullptr; }
+
+rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, const rlc_options *o,
+                   uint32_t *out_labels, rlc_run_stats *stats) {
+  clear_error();
+  if (!ctx || !g) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_run: null context or graph");
+  rlc_status st = check_options(o);
+  if (st != RLC_OK) return st;
+  BusyGuard guard(g->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_run: graph is in use by another call");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const uint64_t n = g->n;
+  bool dense = true;
+  if (init_labels) {
+    for (uint64_t u = 0; u < n; ++u) {
+      if (init_labels[u] == RLC_EMPTY_LABEL) return fail(RLC_ERR_INVALID_ARGUMENT, "init label 0xFFFFFFFF is reserved");
+      dense &= init_labels[u] < n;
+    }
+    if (!g->d_init && (st = dev_alloc(&g->d_init, n)) != RLC_OK) return st;
+    HIP_TRY(hipMemcpyAsync(g->d_init, init_labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+  }
+  const bool sync = o->mode == RLC_MODE_SYNC;
+  if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
+  uint64_t t1, t2;
+  class_thresholds(o, t1, t2);
+  if ((st = ensure_lists(g, t1, t2)) != RLC_OK) return st;
+
+  const DevGraph dg = g->dev();
+  const DevState ds = g->state();
+  const ClassLists cl{g->d_list, g->c1, g->c2, n};
+  LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, o->pickstep, o->tolerance, o->max_iterations, 256};
+  const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
+
+  if (graph_mode && o->max_iterations > 0) {
+    ExecKey k;
+    k.valid = true;
+    k.sync = sync;
+    k.mg = lp.mg;
+    k.pickstep = o->pickstep;
+    k.max_iterations = o->max_iterations;
+    k.tolerance = o->tolerance;
+    k.t1 = t1;
+    k.t2 = t2;
+    if (!(k == g->key)) {
+      if (g->hexec) { (void)hipGraphExecDestroy(g->hexec); g->hexec = nullptr; }
+      if (g->hgraph) { (void)hipGraphDestroy(g->hgraph); g->hgraph = nullptr; }
+      g->key.valid = false;
+      HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+      for (int l = 0; l < o->max_iterations; ++l) launch_iteration(dg, ds, cl, lp, g->stream, nullptr, nullptr);
+      hipGraph_t hg = nullptr;
+      hipError_t ce = hipStreamEndCapture(g->stream, &hg);
+      if (ce != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      g->hgraph = hg;
+      HIP_TRY(hipGraphInstantiate(&g->hexec, g->hgraph, nullptr, nullptr, 0));
+      g->key = k;
+    }
+  }
+
+  float t_total = 0, t_init = 0, t_mark = 0;
+  std::vector<float> iter_ms;
+  for (int r = 0; r < o->repeat; ++r) {
+    iter_ms.clear();
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+    launch_init_labels(dg, ds, init_labels ? g->d_init : nullptr, g->stream);
+    HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+    launch_mark_all(dg, ds, o->max_iterations, g->stream);
+    HIP_TRY(hipEventRecord(g->ev[2], g->stream));
+    if ((st = check_launch()) != RLC_OK) return st;
+    if (graph_mode) {
+      if (o->max_iterations > 0) HIP_TRY(hipGraphLaunch(g->hexec, g->stream));
+    } else {
+      for (int l = 0; l < o->max_iterations; ++l) {
+        launch_iteration(dg, ds, cl, lp, g->stream, g->evm[0], g->evm[1]);
+        if ((st = check_launch()) != RLC_OK) return st;
+        unsigned int done = 0;
+        HIP_TRY(hipMemcpyAsync(&done, &g->d_ctr->done, sizeof(done), hipMemcpyDeviceToHost, g->stream));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, g->evm[0], g->evm[1]));
+        iter_ms.push_back(ms);
+        if (done) break;
+      }
+    }
+    HIP_TRY(hipEventRecord(g->ev[3], g->stream));
+    HIP_TRY(hipEventSynchronize(g->ev[3]));
+    if ((st = check_launch()) != RLC_OK) return st;
+    float a = 0, b = 0, c = 0;
+    HIP_TRY(hipEventElapsedTime(&a, g->ev[0], g->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&b, g->ev[0], g->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&c, g->ev[1], g->ev[2]));
+    t_total += a;
+    t_init += b;
+    t_mark += c;
+  }
+  g->labels_dense = dense;
+
+  DevCounters hc;
+  HIP_TRY(hipMemcpy(&hc, g->d_ctr, sizeof(hc), hipMemcpyDeviceToHost));
+  if (out_labels && n) HIP_TRY(hipMemcpy(out_labels, g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->iterations = (int32_t)hc.iter;
+    stats->time_ms = t_total / o->repeat;
+    stats->init_ms = t_init / o->repeat;
+    stats->marking_ms = t_mark / o->repeat;
+    const uint64_t state_bytes = n * (4 + 1) + (sync ? n * 8 : 0) + sizeof(DevCounters) + (init_labels ? n * 4 : 0);
+    stats->memory_gb = (float)((double)state_bytes / (1024.0 * 1024.0 * 1024.0));
+    stats->changed_last = (uint32_t)hc.last_changed;
+    const uint64_t eb = g->unit ? 8 : 12;
+    uint64_t tb = 0, te = 0;
+    const int L = std::min<int>((int)hc.iter, RLC_MAX_ITER_STATS);
+    for (int l = 0; l < L; ++l) {
+      const uint64_t A = hc.hist[l][0], E = hc.hist[l][1], C = hc.hist[l][2], X = hc.hist[l][3];
+      stats->active[l] = A;
+      stats->edges[l] = E;
+      stats->changed[l] = C;
+      stats->changed_edges[l] = X;
+      te += E;
+      tb += n * 1 + A * (8 + 4 + 1) + E * eb + C * 4 + X * (4 + 1) + (sync ? C * 16 : 0);
+      if (l < (int)iter_ms.size()) stats->iter_ms[l] = iter_ms[l];
+    }
+    stats->traversed_edges = te;
+    stats->algorithmic_bytes = tb;
+  }
+  return RLC_OK;
+}
+
+rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels, double *modularity,
+                          uint64_t *communities) {
+  clear_error();
+  if (!ctx || !g || !modularity) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_modularity: null argument");
+  BusyGuard guard(g->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_modularity: graph is in use by another call");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const uint64_t n = g->n;
+  rlc_status st;
+  if ((!g->d_ctot && (st = dev_alloc(&g->d_ctot, n)) != RLC_OK) ||
+      (!g->d_seen && (st = dev_alloc(&g->d_seen, n)) != RLC_OK) ||
+      (!g->d_modout && (st = dev_alloc(&g->d_modout, 4)) != RLC_OK))
+    return st;
+  const uint32_t *dlab = g->d_lab;
+  std::vector<uint32_t> dense;
+  if (labels || !g->labels_dense) {
+    // labels must index the per-community accumulators: map values >= n to dense ids
+    std::vector<uint32_t> src(n);
+    if (labels) std::copy(labels, labels + n, src.begin());
+    else if (n) HIP_TRY(hipMemcpy(src.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    bool ok = true;
+    for (uint64_t u = 0; u < n; ++u) ok &= src[u] < n;
+    if (!ok) {
+      std::vector<uint32_t> ids;
+      dense.resize(n);
+      std::vector<std::pair<uint32_t, uint32_t>> tmp;  // (label, first position)
+      std::vector<uint32_t> sorted(src);
+      std::sort(sorted.begin(), sorted.end());
+      sorted.erase(std::unique(sorted.begin(), sorted.end()), sorted.end());
+      for (uint64_t u = 0; u < n; ++u)
+        dense[u] = (uint32_t)(std::lower_bound(sorted.begin(), sorted.end(), src[u]) - sorted.begin());
+    } else {
+      dense.swap(src);
+    }
+    if (!g->d_init && (st = dev_alloc(&g->d_init, n)) != RLC_OK) return st;
+    if (n) HIP_TRY(hipMemcpyAsync(g->d_init, dense.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+    dlab = g->d_init;
+  }
+  launch_modularity(g->dev(), dlab, g->d_ctot, g->d_seen, g->d_modout, g->stream);
+  if ((st = check_launch()) != RLC_OK) return st;
+  double out[4];
+  HIP_TRY(hipMemcpyAsync(out, g->d_modout, sizeof(out), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const double W = out[1];  // 2M
+  *modularity = W > 0 ? out[0] / W - out[2] / (W * W) : 0.0;
+  if (communities) *communities = (uint64_t)(out[3] + 0.5);
+  return RLC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,264 @@
+// Seeded synthetic graph generators (host, OpenMP): stochastic block model, R-MAT, road-like grid.
+//
+// The reference reads Matrix-Market files (inc/mtx.hxx:151-187) and symmetrises them
+// (inc/symmetrize.hxx:43-69) into a DiGraph whose adjacency lists are sorted by neighbour id
+// with duplicates removed (inc/_bitset.hxx:245-262). These generators produce the same shape of
+// input directly as CSR: undirected (both directions stored), no self-loops, no duplicates,
+// each adjacency list ascending. Every random draw is a pure function of (seed, index), so the
+// output is identical for any number of threads.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "common.hpp"
+#include "rlc.h"
+
+namespace {
+
+inline uint64_t mix64(uint64_t x) {
+  // splitmix64 finaliser
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+inline uint64_t hash2(uint64_t a, uint64_t b) { return mix64(mix64(a) ^ (b + 0x632BE59BD9B4E019ull)); }
+inline uint64_t hash3(uint64_t a, uint64_t b, uint64_t c) { return hash2(hash2(a, b), c); }
+
+struct Rng {  // splitmix64 stream
+  uint64_t s;
+  explicit Rng(uint64_t seed) : s(seed) {}
+  uint64_t next() { return mix64(s += 0x9E3779B97F4A7C15ull); }
+  double uniform_open0() { return double((next() >> 11) + 1) * 0x1.0p-53; }  // (0, 1]
+};
+
+// Directed-entry builder: gen(i, u, v) yields undirected pair i (returns false to skip it).
+// Two passes over the generator (count, then scatter), then per-vertex sort + unique.
+template <class Gen, class WeightFn>
+rlc_status build_csr(uint64_t n, uint64_t count, Gen gen, WeightFn wfn, bool unit, rlc_csr *out) {
+  std::memset(out, 0, sizeof(*out));
+  out->n = n;
+  uint64_t *deg = (uint64_t *)std::calloc(n + 1, sizeof(uint64_t));
+  if (!deg) return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: degree array");
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)count; ++i) {
+    uint32_t u, v;
+    if (!gen((uint64_t)i, u, v) || u == v) continue;
+    __atomic_fetch_add(&deg[u], 1, __ATOMIC_RELAXED);
+    __atomic_fetch_add(&deg[v], 1, __ATOMIC_RELAXED);
+  }
+  uint64_t *cur = (uint64_t *)std::malloc((n + 1) * sizeof(uint64_t));
+  if (!cur) { std::free(deg); return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: cursor array"); }
+  uint64_t total = 0;
+  for (uint64_t u = 0; u < n; ++u) { cur[u] = total; total += deg[u]; }
+  cur[n] = total;
+  uint32_t *tmp = (uint32_t *)std::malloc(std::max<uint64_t>(total, 1) * sizeof(uint32_t));
+  if (!tmp) { std::free(deg); std::free(cur); return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: edge buffer"); }
+  // deg[] becomes the start offsets (pre-dedup); cur[] the scatter cursors
+  for (uint64_t u = 0; u <= n; ++u) deg[u] = cur[u];
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < (int64_t)count; ++i) {
+    uint32_t u, v;
+    if (!gen((uint64_t)i, u, v) || u == v) continue;
+    tmp[__atomic_fetch_add(&cur[u], 1, __ATOMIC_RELAXED)] = v;
+    tmp[__atomic_fetch_add(&cur[v], 1, __ATOMIC_RELAXED)] = u;
+  }
+  // sort + unique each list; cur[u] = new degree
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t u = 0; u < (int64_t)n; ++u) {
+    uint32_t *b = tmp + deg[u], *e = tmp + deg[u + 1];
+    std::sort(b, e);
+    cur[u] = (uint64_t)(std::unique(b, e) - b);
+  }
+  uint64_t *off = (uint64_t *)std::malloc((n + 1) * sizeof(uint64_t));
+  if (!off) { std::free(deg); std::free(cur); std::free(tmp); return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: offsets"); }
+  uint64_t m = 0;
+  for (uint64_t u = 0; u < n; ++u) { off[u] = m; m += cur[u]; }
+  off[n] = m;
+  uint32_t *edg = (uint32_t *)std::malloc(std::max<uint64_t>(m, 1) * sizeof(uint32_t));
+  float *wei = (float *)std::malloc(std::max<uint64_t>(m, 1) * sizeof(float));
+  if (!edg || !wei) {
+    std::free(deg); std::free(cur); std::free(tmp); std::free(off); std::free(edg); std::free(wei);
+    return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: output arrays");
+  }
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t u = 0; u < (int64_t)n; ++u) {
+    const uint32_t *src = tmp + deg[u];
+    uint64_t o = off[u], d = off[u + 1] - off[u];
+    for (uint64_t j = 0; j < d; ++j) {
+      uint32_t v = src[j];
+      edg[o + j] = v;
+      wei[o + j] = unit ? 1.0f : wfn((uint32_t)u, v);
+    }
+  }
+  std::free(deg); std::free(cur); std::free(tmp);
+  out->m = m;
+  out->offsets = off;
+  out->edges = edg;
+  out->weights = wei;
+  return RLC_OK;
+}
+
+// Geometric skipping over `len` Bernoulli(p) trials (Batagelj & Brandes 2005): calls emit(k)
+// for each success index k in increasing order.
+template <class Emit>
+void bernoulli_skip(Rng &r, uint64_t len, double p, Emit emit) {
+  if (len == 0 || p <= 0) return;
+  if (p >= 1) { for (uint64_t k = 0; k < len; ++k) emit(k); return; }
+  const double lq = std::log1p(-p);
+  uint64_t pos = 0;  // next candidate index
+  for (;;) {
+    double s = std::floor(std::log(r.uniform_open0()) / lq);  // failures before the next success
+    if (!(s < double(len - pos))) return;
+    uint64_t k = pos + (uint64_t)s;
+    if (k >= len) return;
+    emit(k);
+    pos = k + 1;
+    if (pos >= len) return;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+rlc_status rlc_gen_sbm(uint64_t n, uint32_t blocks, double degree, double intra, uint64_t seed, rlc_csr *out) {
+  rlc::clear_error();
+  if (!out) return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_sbm: out is null");
+  if (n == 0 || n >= 0xFFFFFFFFull || blocks == 0 || blocks > n || !(degree >= 0) || !(intra >= 0 && intra <= 1))
+    return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_sbm: bad parameters");
+  const uint64_t B = blocks;
+  auto bstart = [&](uint64_t b) { return (b * n) / B; };
+  const double avg = double(n) / double(B);
+  const double pin = avg > 1 ? std::min(1.0, intra * degree / (avg - 1)) : 0.0;
+  const double pout = (n > avg) ? std::min(1.0, (1 - intra) * degree / (double(n) - avg)) : 0.0;
+  // task b: the intra-block triangle of block b, then the inter-block rectangles (b, b2 > b);
+  // each triangle/rectangle draws from its own stream, seeded by its block pair
+  int T = omp_get_max_threads();
+  std::vector<std::vector<uint64_t>> parts(T);
+#pragma omp parallel
+  {
+    std::vector<uint64_t> &mine = parts[omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t bi = 0; bi < (int64_t)B; ++bi) {
+      const uint64_t b1 = (uint64_t)bi;
+      const uint64_t a0 = bstart(b1), sa = bstart(b1 + 1) - a0;
+      {
+        Rng r(hash2(seed, b1 * B + b1));
+        uint64_t len = sa * (sa - 1) / 2;
+        uint64_t row = 0, rs = 0, rl = sa - 1;  // row i holds pairs (i, i+1..sa-1)
+        bernoulli_skip(r, len, pin, [&](uint64_t k) {
+          while (k >= rs + rl) { rs += rl; ++row; --rl; }
+          uint64_t i = row, j = row + 1 + (k - rs);
+          mine.push_back(((a0 + i) << 32) | (a0 + j));
+        });
+      }
+      for (uint64_t b2 = b1 + 1; b2 < B; ++b2) {
+        Rng r(hash2(seed, b1 * B + b2));
+        uint64_t c0 = bstart(b2), sc = bstart(b2 + 1) - c0;
+        bernoulli_skip(r, sa * sc, pout, [&](uint64_t k) {
+          mine.push_back(((a0 + k / sc) << 32) | (c0 + k % sc));
+        });
+      }
+    }
+  }
+  std::vector<uint64_t> pairs;
+  size_t tot = 0;
+  for (auto &p : parts) tot += p.size();
+  try { pairs.reserve(tot); } catch (...) { return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "rlc_gen_sbm: pair list"); }
+  for (auto &p : parts) { pairs.insert(pairs.end(), p.begin(), p.end()); std::vector<uint64_t>().swap(p); }
+  auto gen = [&](uint64_t i, uint32_t &u, uint32_t &v) {
+    u = (uint32_t)(pairs[i] >> 32); v = (uint32_t)pairs[i];
+    return true;
+  };
+  auto wfn = [](uint32_t, uint32_t) { return 1.0f; };
+  return build_csr(n, pairs.size(), gen, wfn, true, out);
+}
+
+rlc_status rlc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, float wmin, float wmax,
+                        uint64_t seed, rlc_csr *out) {
+  rlc::clear_error();
+  if (!out) return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_rmat: out is null");
+  if (scale == 0 || scale > 31 || edge_factor == 0 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0 + 1e-12 ||
+      !(wmin > 0) || wmax < wmin)
+    return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_rmat: bad parameters");
+  const uint64_t n = 1ull << scale;
+  const uint64_t count = (uint64_t)edge_factor << scale;
+  const uint64_t ta = (uint64_t)(a * 4294967296.0), tab = (uint64_t)((a + b) * 4294967296.0),
+                 tabc = (uint64_t)((a + b + c) * 4294967296.0);
+  auto gen = [=](uint64_t i, uint32_t &u, uint32_t &v) {
+    uint64_t s = hash2(seed, i);
+    uint32_t x = 0, y = 0;
+    uint64_t bits = 0;
+    for (uint32_t l = 0; l < scale; ++l) {
+      if ((l & 1) == 0) bits = mix64(s + l);
+      uint64_t r = (l & 1) ? (bits >> 32) : (bits & 0xFFFFFFFFull);
+      uint32_t bx = r >= tab, by = (r >= ta && r < tab) || r >= tabc;
+      x = (x << 1) | bx;
+      y = (y << 1) | by;
+    }
+    u = x; v = y;
+    return true;
+  };
+  const bool unit = (wmin == wmax);
+  const float wspan = wmax - wmin;
+  const uint64_t wseed = mix64(seed ^ 0x5157A1D3ull);
+  auto wfn = [=](uint32_t u, uint32_t v) {
+    uint32_t lo = u < v ? u : v, hi = u < v ? v : u;
+    double f = double(hash3(wseed, lo, hi) >> 40) * 0x1.0p-24;
+    return (float)(double(wmin) + double(wspan) * f);
+  };
+  rlc_status st = build_csr(n, count, gen, wfn, unit, out);
+  if (st == RLC_OK && unit && wmin != 1.0f)
+    for (uint64_t i = 0; i < out->m; ++i) out->weights[i] = wmin;
+  return st;
+}
+
+rlc_status rlc_gen_grid(uint32_t rows, uint32_t cols, double drop, double shortcut, uint64_t seed, rlc_csr *out) {
+  rlc::clear_error();
+  if (!out) return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_grid: out is null");
+  const uint64_t n = (uint64_t)rows * cols;
+  if (rows == 0 || cols == 0 || n >= 0xFFFFFFFFull || !(drop >= 0 && drop <= 1) || !(shortcut >= 0 && shortcut <= 1))
+    return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_grid: bad parameters");
+  // pair index space: [0, n) right-links, [n, 2n) down-links, [2n, 3n) shortcut candidates;
+  // draws compare the top 53 bits of a hash against p * 2^53
+  const uint64_t keep = (uint64_t)((1.0 - drop) * 9007199254740992.0);
+  const uint64_t take = (uint64_t)(shortcut * 9007199254740992.0);
+  auto gen = [=](uint64_t i, uint32_t &u, uint32_t &v) {
+    uint64_t h = hash2(seed, i);
+    uint64_t h53 = h >> 11;
+    if (i < n) {
+      uint64_t r = i / cols, c = i % cols;
+      if (c + 1 >= cols || h53 >= keep) return false;
+      u = (uint32_t)i; v = (uint32_t)(r * cols + c + 1);
+      return true;
+    }
+    if (i < 2 * n) {
+      uint64_t k = i - n, r = k / cols;
+      if (r + 1 >= rows || h53 >= keep) return false;
+      u = (uint32_t)k; v = (uint32_t)(k + cols);
+      return true;
+    }
+    uint64_t k = i - 2 * n;
+    if (h53 >= take) return false;
+    u = (uint32_t)k; v = (uint32_t)(mix64(h) % n);
+    return true;
+  };
+  auto wfn = [](uint32_t, uint32_t) { return 1.0f; };
+  return build_csr(n, 3 * n, gen, wfn, true, out);
+}
+
+void rlc_csr_release(rlc_csr *g) {
+  if (!g) return;
+  std::free(g->offsets);
+  std::free(g->edges);
+  std::free(g->weights);
+  std::memset(g, 0, sizeof(*g));
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Host-side helpers (g++ / OpenMP): error state, CSR validation, degree-class lists.
+#include "host.hpp"
+
+#include <omp.h>
+
+#include <string>
+
+namespace rlc {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+const char *last_error_cstr() { return g_last_error.c_str(); }
+
+rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
+                        uint64_t *max_degree) {
+  if (n >= 0xFFFFFFFFull) return fail(RLC_ERR_INVALID_GRAPH, "graph: n must be below 2^32-1 (label 0xFFFFFFFF is reserved)");
+  if (!off) return fail(RLC_ERR_INVALID_ARGUMENT, "graph: offsets is null");
+  if (m > 0 && !edg) return fail(RLC_ERR_INVALID_ARGUMENT, "graph: edges is null");
+  if (off[0] != 0) return fail(RLC_ERR_INVALID_GRAPH, "graph: offsets[0] != 0");
+  if (off[n] != m) return fail(RLC_ERR_INVALID_GRAPH, "graph: offsets[n] != m");
+  int bad_off = 0, bad_edge = 0, bad_w = 0;
+  uint64_t md = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad_off) reduction(max : md)
+  for (int64_t u = 0; u < (int64_t)n; ++u) {
+    if (off[u + 1] < off[u]) { bad_off = 1; continue; }
+    uint64_t d = off[u + 1] - off[u];
+    md = d > md ? d : md;
+  }
+  if (bad_off) return fail(RLC_ERR_INVALID_GRAPH, "graph: offsets are not non-decreasing");
+#pragma omp parallel for schedule(static) reduction(| : bad_edge, bad_w)
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    bad_edge |= edg[i] >= n;
+    if (wei) bad_w |= !(wei[i] >= 0.0f) || !(wei[i] < __builtin_huge_valf());
+  }
+  if (bad_edge) return fail(RLC_ERR_INVALID_GRAPH, "graph: an edge target is >= n");
+  if (bad_w) return fail(RLC_ERR_INVALID_GRAPH, "graph: weights must be finite and non-negative");
+  if (max_degree) *max_degree = md;
+  return RLC_OK;
+}
+
+void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, std::vector<uint32_t> &list,
+                       uint64_t &c1, uint64_t &c2) {
+  list.resize(n);
+  uint64_t k0 = 0, k1 = 0;
+  for (uint64_t u = 0; u < n; ++u) {
+    uint64_t d = off[u + 1] - off[u];
+    if (d < t1) ++k0;
+    else if (d < t2) ++k1;
+  }
+  uint64_t p0 = 0, p1 = k0, p2 = k0 + k1;
+  for (uint64_t u = 0; u < n; ++u) {
+    uint64_t d = off[u + 1] - off[u];
+    if (d < t1) list[p0++] = (uint32_t)u;
+    else if (d < t2) list[p1++] = (uint32_t)u;
+    else list[p2++] = (uint32_t)u;
+  }
+  c1 = k0;
+  c2 = k0 + k1;
+}
+
+}  // namespace rlc

@@ -1,0 +1,105 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ from the COMPILED REFERENCE (oracle/_ref/libref.so,
+built by `make -C oracle` from the reference's own CPU headers). Run in the container where the
+reference tree is present:
+
+    python tests/golden/make_fixtures.py
+
+Each fixture is an .npz (no pickles) holding the input CSR (or, for the 65,536-vertex
+configuration, its generator parameters and a checksum) and the reference outputs:
+  lowmem_mg8 / lowmem_bm   rakLowmemStaticOmp<true|false, true, 8>, one thread: labels, iterations
+  rak                      rakStaticOmp (GVE-LPA), one thread: labels, iterations
+  sync_*                   the synchronous mode driven through the reference's per-vertex functions
+  modularity / communities modularityBy / communities of the MG8 labels
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "rak-lowmem-communities-cuda_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle as O  # noqa: E402
+import rlc  # noqa: E402
+from graphs import cliques, from_pairs, random_weighted, star, with_hubs  # noqa: E402
+
+SYNC_SETTINGS = [  # (name, slots, pickstep, merge_degree, merge_parts)
+    ("sync_mg8", 8, 8, 0, 64),
+    ("sync_bm", 1, 8, 0, 64),
+    ("sync_mg8_p0", 8, 0, 0, 64),
+    ("sync_mg8_m64", 8, 8, 6, 64),
+    ("sync_mg8_m256", 8, 8, 12, 256),
+    ("sync_bm_m64", 1, 8, 6, 64),
+]
+
+
+def csr_digest(g):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(g.offsets, np.uint64).tobytes())
+    h.update(np.ascontiguousarray(g.edges, np.uint32).tobytes())
+    if g.weights is not None:
+        h.update(np.ascontiguousarray(g.weights, np.float32).tobytes())
+    return h.hexdigest()
+
+
+def reference_outputs(g, sync_settings):
+    out = {}
+    for multi, name in ((True, "lowmem_mg8"), (False, "lowmem_bm")):
+        lab, it, _ = O.ref_lowmem_omp(g, multi, threads=1)
+        out[name + "_labels"] = lab
+        out[name + "_iterations"] = np.int32(it)
+    lab, it, _ = O.ref_rak_omp(g, threads=1)
+    out["rak_labels"] = lab
+    out["rak_iterations"] = np.int32(it)
+    for name, slots, pick, md, mp in sync_settings:
+        lab, it, st = O.ref_lpa_sync(g, slots=slots, pickstep=pick, merge_degree=md, merge_parts=mp)
+        out[name + "_labels"] = lab
+        out[name + "_iterations"] = np.int32(it)
+        out[name + "_stats"] = st
+    mg = out["lowmem_mg8_labels"]
+    out["modularity"] = np.float64(O.ref_modularity(g, mg))
+    out["communities"] = np.int64(O.ref_communities(g, mg))
+    return out
+
+
+def save(name, g, outs, store_graph=True, gen=None):
+    d = dict(outs)
+    if store_graph:
+        d["offsets"] = np.asarray(g.offsets, np.uint64)
+        d["edges"] = np.asarray(g.edges, np.uint32)
+        if g.weights is not None:
+            d["weights"] = np.asarray(g.weights, np.float32)
+    if gen is not None:
+        d["generator"] = np.array(gen)
+    d["csr_sha256"] = np.array(csr_digest(g))
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **d)
+    print("wrote", name, g.n, g.m)
+
+
+def main():
+    if not O.ref_available():
+        sys.exit("oracle/_ref/libref.so is not built: run `make -C oracle` where the reference tree is present")
+    small = {
+        "sbm_2k": rlc.gen_sbm(2048, 16, 10.0, 0.8, 21),
+        "weighted_1k": random_weighted(1000, 5000, 5),
+        "hubs_1500": with_hubs(1500, hubs=4, hub_degree=700, base_degree=10, seed=3),
+        "rmat_10w": rlc.gen_rmat(10, 16, wmin=0.5, wmax=1.5, seed=4),
+        "cliques": cliques([5, 8, 13, 21]),
+        "star_200": star(200),
+        "self_loops": from_pairs(6, [(0, 0), (0, 1), (1, 1), (1, 2), (3, 3), (4, 5)]),
+    }
+    for name, g in small.items():
+        save(name, g, reference_outputs(g, SYNC_SETTINGS))
+    # the plain MG8 configuration: 65,536 vertices, 256 blocks, degree 16, 80% intra-block, seed 1
+    g = rlc.gen_sbm(65536, 256, 16.0, 0.8, 1)
+    save("sbm_65536", g, reference_outputs(g, SYNC_SETTINGS[:3]), store_graph=False,
+         gen=["sbm", 65536, 256, 16.0, 0.8, 1])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,77 @@
+"""Small hand-built and seeded graphs shared by the tests (host CSR, same layout as rlc.Graph)."""
+from __future__ import annotations
+
+import numpy as np
+
+import rlc
+
+
+def from_pairs(n, pairs, weights=None, symmetric=True, keep_self=True):
+    """CSR from (u, v) pairs; symmetric adds (v, u); adjacency sorted by target, duplicates kept
+    only once (the last weight wins, as DiGraph.addEdge does)."""
+    d = {}
+    for k, (u, v) in enumerate(pairs):
+        w = 1.0 if weights is None else float(weights[k])
+        if u == v and not keep_self:
+            continue
+        d[(u, v)] = w
+        if symmetric:
+            d[(v, u)] = w
+    rows = [[] for _ in range(n)]
+    for (u, v), w in d.items():
+        rows[u].append((v, w))
+    off = np.zeros(n + 1, np.uint64)
+    edg, wei = [], []
+    for u in range(n):
+        rows[u].sort()
+        off[u + 1] = off[u] + len(rows[u])
+        for v, w in rows[u]:
+            edg.append(v)
+            wei.append(w)
+    return rlc.Graph(off, np.array(edg, np.uint32), None if weights is None else np.array(wei, np.float32))
+
+
+def cfg1():
+    """Plain MG8-LPA configuration: SBM, 65,536 vertices, 256 blocks, degree 16, 80% intra."""
+    return rlc.gen_sbm(65536, 256, 16.0, 0.8, 1)
+
+
+def star(k):
+    return from_pairs(k + 1, [(0, i) for i in range(1, k + 1)])
+
+
+def cliques(sizes, bridges=True):
+    pairs, base = [], 0
+    starts = []
+    for s in sizes:
+        starts.append(base)
+        pairs += [(base + i, base + j) for i in range(s) for j in range(i + 1, s)]
+        base += s
+    if bridges:
+        pairs += [(starts[i], starts[i + 1]) for i in range(len(starts) - 1)]
+    return from_pairs(base, pairs)
+
+
+def random_weighted(n, m, seed, self_loops=True):
+    r = np.random.default_rng(seed)
+    u = r.integers(0, n, m)
+    v = r.integers(0, n, m)
+    if not self_loops:
+        keep = u != v
+        u, v = u[keep], v[keep]
+    w = r.uniform(0.5, 1.5, len(u)).astype(np.float32)
+    return from_pairs(n, list(zip(u.tolist(), v.tolist())), w)
+
+
+def with_hubs(n, hubs, hub_degree, base_degree, seed, weighted=True):
+    """Random graph where `hubs` vertices have about `hub_degree` neighbours each."""
+    r = np.random.default_rng(seed)
+    pairs = []
+    for h in range(hubs):
+        for v in r.choice(n, size=min(hub_degree, n - 1), replace=False):
+            if v != h:
+                pairs.append((h, int(v)))
+    m = n * base_degree // 2
+    pairs += list(zip(r.integers(0, n, m).tolist(), r.integers(0, n, m).tolist()))
+    w = r.uniform(0.5, 1.5, len(pairs)).astype(np.float32) if weighted else None
+    return from_pairs(n, pairs, w)

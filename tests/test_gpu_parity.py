@@ -1,0 +1,193 @@
+"""Device parity: the HIP path (through the C ABI) against the oracle on the same seeded inputs.
+
+Synchronous mode is deterministic and must match the oracle bit for bit (labels, iteration count
+and the per-iteration counters). Asynchronous mode updates labels in place like the reference and
+is racy by design, so it is held to the quality gate: FP64 modularity within 1% of the reference
+CPU MG8 path (rakLowmemStaticOmp<true,true,8>, restated in the oracle and pinned to the compiled
+reference), and labels that are always labels some vertex started with.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rlc
+from graphs import cfg1, cliques, from_pairs, random_weighted, star, with_hubs
+
+pytestmark = pytest.mark.gpu
+
+
+def run_sync(ctx, g, **kw):
+    init = kw.pop("init", None)
+    with ctx.upload(g) as dg:
+        r = dg.run(rlc.options(mode=rlc.MODE_SYNC, **kw), init=init)
+        return r, rlc.iteration_stats(r.stats)
+
+
+def oracle_sync(g, **kw):
+    slots = kw.get("slots", 8)
+    return O.lpa_sync(g, slots=slots, tolerance=kw.get("tolerance", 0.05), max_iterations=kw.get("max_iterations", 20),
+                      pickstep=kw.get("pickstep", 8), merge_degree=kw.get("merge_degree", 0),
+                      merge_parts=kw.get("merge_parts", 64), init=kw.get("init"))
+
+
+def assert_sync_parity(ctx, g, **kw):
+    print(f"sync parity n={g.n} m={g.m} opts={kw}", flush=True)
+    r, st = run_sync(ctx, g, **dict(kw))
+    lab, it, ost = oracle_sync(g, **kw)
+    assert r.iterations == it
+    assert np.array_equal(r.membership, lab)
+    assert np.array_equal(st, ost[: len(st)])
+
+
+@pytest.mark.parametrize("slots", [8, 1])
+def test_cfg1_sync_bit_exact(ctx, slots):
+    g = cfg1()
+    r, st = run_sync(ctx, g, slots=slots)
+    lab, it, ost = oracle_sync(g, slots=slots)
+    assert r.iterations == it
+    assert np.array_equal(r.membership, lab)
+    assert np.array_equal(st, ost)
+    # traversed edges = sum over iterations of the processed degrees
+    assert r.stats.traversed_edges == int(ost[:, 1].sum())
+
+
+@pytest.mark.parametrize("slots", [8, 1])
+@pytest.mark.parametrize("merge", [(0, 64), (8, 64), (40, 256), (1, 64), (1, 256)])
+def test_sync_merge_tree_bit_exact(ctx, slots, merge):
+    g = with_hubs(3000, hubs=6, hub_degree=1500, base_degree=12, seed=7)
+    assert_sync_parity(ctx, g, slots=slots, merge_degree=merge[0], merge_parts=merge[1])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sync_random_weighted(ctx, seed):
+    g = random_weighted(2000, 9000, seed)
+    assert_sync_parity(ctx, g, slots=8)
+    assert_sync_parity(ctx, g, slots=8, merge_degree=10, merge_parts=64)
+    assert_sync_parity(ctx, g, slots=1)
+
+
+def test_sync_rmat_weighted(ctx):
+    g = rlc.gen_rmat(13, 16, wmin=0.5, wmax=1.5, seed=3)
+    for md, mp in [(0, 64), (64, 64), (512, 256)]:
+        assert_sync_parity(ctx, g, slots=8, merge_degree=md, merge_parts=mp)
+
+
+@pytest.mark.parametrize("pickstep", [0, 1, 3, 8])
+def test_sync_pickstep(ctx, pickstep):
+    g = rlc.gen_sbm(5000, 40, 10.0, 0.7, 11)
+    assert_sync_parity(ctx, g, slots=8, pickstep=pickstep)
+
+
+def test_sync_tolerance_and_caps(ctx):
+    g = rlc.gen_sbm(4000, 20, 8.0, 0.8, 5)
+    assert_sync_parity(ctx, g, tolerance=0.0, max_iterations=30)
+    assert_sync_parity(ctx, g, max_iterations=1)
+    r, st = run_sync(ctx, g, max_iterations=0)
+    assert r.iterations == 0
+    assert np.array_equal(r.membership, np.arange(g.n, dtype=np.uint32))
+
+
+def test_sync_custom_init(ctx):
+    g = rlc.gen_sbm(3000, 30, 10.0, 0.8, 9)
+    rng = np.random.default_rng(0)
+    init = rng.integers(0, 2**32 - 2, g.n, dtype=np.uint64).astype(np.uint32)
+    assert_sync_parity(ctx, g, init=init)
+    init2 = (np.arange(g.n) // 50).astype(np.uint32)  # resume from a coarse partition
+    assert_sync_parity(ctx, g, init=init2)
+
+
+def test_edge_cases(ctx):
+    cases = {
+        "single": from_pairs(1, []),
+        "isolated": from_pairs(10, [(0, 1), (2, 3)]),
+        "self_loops": from_pairs(6, [(0, 0), (0, 1), (1, 1), (1, 2), (3, 3), (4, 5)]),
+        "path": from_pairs(50, [(i, i + 1) for i in range(49)]),
+        "star": star(300),
+        "cliques": cliques([5, 8, 13, 21]),
+        "complete": cliques([40], bridges=False),
+    }
+    for name, g in cases.items():
+        print("case", name, flush=True)
+        for slots in (8, 1):
+            assert_sync_parity(ctx, g, slots=slots)
+            assert_sync_parity(ctx, g, slots=slots, merge_degree=2, merge_parts=64)
+            assert_sync_parity(ctx, g, slots=slots, merge_degree=3, merge_parts=256)
+
+
+def test_cliques_converge_and_isolated_stay(ctx):
+    g = cliques([6, 9, 12], bridges=False)
+    r, _ = run_sync(ctx, g, pickstep=0)
+    starts = [0, 6, 15]
+    for s, k in zip(starts, [6, 9, 12]):
+        assert len(set(r.membership[s : s + k].tolist())) == 1
+    g2 = from_pairs(5, [(0, 1)])
+    r2, _ = run_sync(ctx, g2)
+    assert r2.membership[2:].tolist() == [2, 3, 4]
+
+
+def test_offsets32_upload_matches(ctx):
+    g = rlc.gen_sbm(3000, 30, 10.0, 0.8, 4)
+    with ctx.upload(g, offsets32=True) as dg:
+        r = dg.run(rlc.options(mode=rlc.MODE_SYNC))
+    lab, it, _ = oracle_sync(g)
+    assert r.iterations == it and np.array_equal(r.membership, lab)
+
+
+def test_sync_repeat_and_rerun_deterministic(ctx):
+    g = rlc.gen_rmat(12, 16, wmin=0.5, wmax=1.5, seed=8)
+    with ctx.upload(g) as dg:
+        a = dg.run(rlc.options(mode=rlc.MODE_SYNC, merge_degree=64))
+        b = dg.run(rlc.options(mode=rlc.MODE_SYNC, merge_degree=64, repeat=3))
+        c = dg.run(rlc.options(mode=rlc.MODE_SYNC, merge_degree=64, timing=rlc.TIMING_PER_ITERATION))
+    assert np.array_equal(a.membership, b.membership) and np.array_equal(a.membership, c.membership)
+    assert a.iterations == b.iterations == c.iterations
+    assert all(c.stats.iter_ms[i] > 0 for i in range(c.iterations))
+
+
+# ---- asynchronous mode: quality gate ----------------------------------------------------------------
+def async_quality(ctx, g, slots=8, **kw):
+    with ctx.upload(g) as dg:
+        r = dg.run(rlc.options(mode=rlc.MODE_ASYNC, slots=slots, **kw))
+        q_dev, k_dev = dg.modularity()
+    q_host = O.modularity(g, r.membership)
+    assert abs(q_dev - q_host) < 1e-9
+    assert k_dev == O.communities(r.membership)
+    # labels are vertex ids that exist (initial labels = ids)
+    assert r.membership.max() < g.n
+    return r, q_dev
+
+
+@pytest.mark.parametrize("wave,block", [(0, 0), (4, 64), (1, 1 << 30), (1 << 30, 1 << 30), (2, 2)])
+def test_async_cfg1_quality(ctx, wave, block):
+    g = cfg1()
+    r, q = async_quality(ctx, g, wave_degree=wave, block_degree=block)
+    q_ref = O.modularity(g, O.lowmem_omp(g, True)[0])
+    print(f"cfg1 async wave={wave} block={block}: Q={q:.6f} ref={q_ref:.6f} iters={r.iterations}")
+    assert q >= q_ref * 0.99
+
+
+def test_async_rmat_quality(ctx):
+    g = rlc.gen_rmat(15, 16, wmin=0.5, wmax=1.5, seed=2)
+    r, q = async_quality(ctx, g)
+    q_ref = O.modularity(g, O.lowmem_omp(g, True)[0])
+    print(f"rmat15 async: Q={q:.6f} ref={q_ref:.6f} iters={r.iterations}")
+    assert q >= q_ref - 0.01 * abs(q_ref)
+
+
+def test_async_bm_runs(ctx):
+    g = cfg1()
+    r, q = async_quality(ctx, g, slots=1)
+    q_ref = O.modularity(g, O.lowmem_omp(g, False)[0])
+    print(f"cfg1 async BM: Q={q:.6f} ref-BM={q_ref:.6f}")
+    assert q > 0.5 * q_ref
+
+
+def test_modularity_device_matches_oracle(ctx):
+    g = rlc.gen_rmat(12, 8, wmin=0.5, wmax=1.5, seed=4)
+    rng = np.random.default_rng(1)
+    with ctx.upload(g) as dg:
+        for labels in [np.arange(g.n, dtype=np.uint32), (np.arange(g.n) % 7).astype(np.uint32),
+                       rng.integers(0, 2**32 - 2, g.n, dtype=np.uint64).astype(np.uint32)]:
+            q, k = dg.modularity(labels)
+            assert abs(q - O.modularity(g, labels)) < 1e-9
+            assert k == O.communities(labels)
