@@ -31,6 +31,7 @@ extern "C" {
 
 #define RLC_ABI_VERSION 1
 #define RLC_MAX_ITER_STATS 64
+#define RLC_MAX_SWEEPS 64
 #define RLC_EMPTY_LABEL 0xFFFFFFFFu /* reserved: never a valid label */
 
 typedef enum rlc_status {
@@ -78,7 +79,11 @@ typedef struct rlc_options {
   int32_t wave_degree;    /* ASYNC: degree >= wave_degree -> one wavefront per vertex (0 = library default) */
   int32_t block_degree;   /* ASYNC: degree >= block_degree -> one workgroup per vertex (0 = library default) */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
-  int32_t reserved[7];    /* must be zero */
+  int32_t sweeps;         /* ASYNC: each iteration visits the vertices in S sub-sweeps, one launch
+                             group each, vertex u in sub-sweep (u / 64) % S, so that later
+                             sub-sweeps see the labels written by earlier ones (0 = library
+                             default: up to 8, at least 4096 vertices per sub-sweep). SYNC: ignored */
+  int32_t reserved[6];    /* must be zero */
 } rlc_options;
 
 typedef struct rlc_run_stats {

@@ -34,15 +34,18 @@ namespace {
 constexpr uint64_t kNoLimit = ~0ull;
 constexpr int kDefaultWaveDegree = 32;
 constexpr int kDefaultBlockDegree = 2048;
+constexpr int kDefaultSweeps = 8;
+constexpr uint64_t kMinSweepVertices = 4096;
 
 struct ExecKey {
   bool valid = false;
   int sync, mg, pickstep, max_iterations;
   double tolerance;
   uint64_t t1, t2;
+  int sweeps;
   bool operator==(const ExecKey &o) const {
     return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
-           max_iterations == o.max_iterations && tolerance == o.tolerance && t1 == o.t1 && t2 == o.t2;
+           max_iterations == o.max_iterations && tolerance == o.tolerance && t1 == o.t1 && t2 == o.t2 && sweeps == o.sweeps;
   }
 };
 
@@ -70,7 +73,9 @@ struct rlc_graph {
   float *d_wei = nullptr;
   // degree classes
   uint32_t *d_list = nullptr;
-  uint64_t list_t1 = 0, list_t2 = 0, c1 = 0, c2 = 0;
+  uint64_t list_t1 = 0, list_t2 = 0;
+  int list_sweeps = 0;
+  std::vector<uint64_t> seg;
   bool list_valid = false;
   // state
   uint32_t *d_lab = nullptr;
@@ -124,11 +129,11 @@ void free_graph(rlc_graph *g) {
   delete g;
 }
 
-rlc_status ensure_lists(rlc_graph *g, uint64_t t1, uint64_t t2) {
-  if (g->list_valid && g->list_t1 == t1 && g->list_t2 == t2) return RLC_OK;
+rlc_status ensure_lists(rlc_graph *g, uint64_t t1, uint64_t t2, int sweeps) {
+  if (g->list_valid && g->list_t1 == t1 && g->list_t2 == t2 && g->list_sweeps == sweeps) return RLC_OK;
   std::vector<uint32_t> list;
-  uint64_t c1, c2;
-  build_class_lists(g->n, g->h_off.data(), t1, t2, list, c1, c2);
+  std::vector<uint64_t> seg;
+  build_class_lists(g->n, g->h_off.data(), t1, t2, sweeps, list, seg);
   if (!g->d_list) {
     rlc_status st = dev_alloc(&g->d_list, g->n);
     if (st != RLC_OK) return st;
@@ -137,8 +142,8 @@ rlc_status ensure_lists(rlc_graph *g, uint64_t t1, uint64_t t2) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->list_t1 = t1;
   g->list_t2 = t2;
-  g->c1 = c1;
-  g->c2 = c2;
+  g->list_sweeps = sweeps;
+  g->seg.swap(seg);
   g->list_valid = true;
   g->key.valid = false;  // captured launches embed the class sizes
   return RLC_OK;
@@ -159,11 +164,21 @@ rlc_status check_options(const rlc_options *o) {
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts must be 64 or 256");
   if (o->wave_degree < 0 || o->block_degree < 0)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.wave_degree / block_degree must be >= 0");
+  if (o->sweeps < 0 || o->sweeps > RLC_MAX_SWEEPS)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.sweeps must be in [0, RLC_MAX_SWEEPS]");
   if (o->timing != RLC_TIMING_GRAPH && o->timing != RLC_TIMING_PER_ITERATION)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.timing must be RLC_TIMING_GRAPH or RLC_TIMING_PER_ITERATION");
   for (int r : o->reserved)
     if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.reserved must be zero");
   return RLC_OK;
+}
+
+// sub-sweeps of an asynchronous iteration (the synchronous mode reads a snapshot: one sweep)
+int sweep_count(const rlc_options *o, uint64_t n) {
+  if (o->mode == RLC_MODE_SYNC) return 1;
+  if (o->sweeps > 0) return o->sweeps;
+  uint64_t s = n / kMinSweepVertices;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(kDefaultSweeps, s));
 }
 
 // degree thresholds (t1, t2) of the lane / wave / workgroup classes
@@ -348,11 +363,12 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
   uint64_t t1, t2;
   class_thresholds(o, t1, t2);
-  if ((st = ensure_lists(g, t1, t2)) != RLC_OK) return st;
+  const int sweeps = sweep_count(o, n);
+  if ((st = ensure_lists(g, t1, t2, sweeps)) != RLC_OK) return st;
 
   const DevGraph dg = g->dev();
   const DevState ds = g->state();
-  const ClassLists cl{g->d_list, g->c1, g->c2, n};
+  const ClassLists cl{g->d_list, g->seg.data(), sweeps};
   LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, o->pickstep, o->tolerance, o->max_iterations, 256};
   const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
 
@@ -366,6 +382,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     k.tolerance = o->tolerance;
     k.t1 = t1;
     k.t2 = t2;
+    k.sweeps = sweeps;
     if (!(k == g->key)) {
       if (g->hexec) { (void)hipGraphExecDestroy(g->hexec); g->hexec = nullptr; }
       if (g->hgraph) { (void)hipGraphDestroy(g->hgraph); g->hgraph = nullptr; }

@@ -42,24 +42,20 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
   return RLC_OK;
 }
 
-void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, std::vector<uint32_t> &list,
-                       uint64_t &c1, uint64_t &c2) {
+void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, int sweeps,
+                       std::vector<uint32_t> &list, std::vector<uint64_t> &seg) {
+  const uint64_t S = (uint64_t)(sweeps < 1 ? 1 : sweeps);
   list.resize(n);
-  uint64_t k0 = 0, k1 = 0;
-  for (uint64_t u = 0; u < n; ++u) {
-    uint64_t d = off[u + 1] - off[u];
-    if (d < t1) ++k0;
-    else if (d < t2) ++k1;
-  }
-  uint64_t p0 = 0, p1 = k0, p2 = k0 + k1;
-  for (uint64_t u = 0; u < n; ++u) {
-    uint64_t d = off[u + 1] - off[u];
-    if (d < t1) list[p0++] = (uint32_t)u;
-    else if (d < t2) list[p1++] = (uint32_t)u;
-    else list[p2++] = (uint32_t)u;
-  }
-  c1 = k0;
-  c2 = k0 + k1;
+  seg.assign(3 * S + 1, 0);
+  auto group = [&](uint64_t u) {
+    const uint64_t d = off[u + 1] - off[u];
+    const uint64_t c = d < t1 ? 0 : (d < t2 ? 1 : 2);
+    return c * S + (u / 64) % S;
+  };
+  for (uint64_t u = 0; u < n; ++u) ++seg[group(u) + 1];
+  for (uint64_t k = 0; k < 3 * S; ++k) seg[k + 1] += seg[k];
+  std::vector<uint64_t> pos(seg.begin(), seg.end() - 1);
+  for (uint64_t u = 0; u < n; ++u) list[pos[group(u)]++] = (uint32_t)u;
 }
 
 }  // namespace rlc

@@ -12,8 +12,10 @@ const char *last_error_cstr();
 rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
                         uint64_t *max_degree);
 
-// Stable 3-way partition of 0..n-1 by degree: [deg < t1 | t1 <= deg < t2 | deg >= t2].
-void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, std::vector<uint32_t> &list,
-                       uint64_t &c1, uint64_t &c2);
+// Vertex order of one iteration: grouped by degree class (0: deg < t1, 1: t1 <= deg < t2,
+// 2: deg >= t2), then by sub-sweep (u / 64) % sweeps, ascending id inside each group.
+// seg[c * sweeps + s] is the first position of (class c, sub-sweep s); seg[3 * sweeps] = n.
+void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, int sweeps,
+                       std::vector<uint32_t> &list, std::vector<uint64_t> &seg);
 
 }  // namespace rlc

@@ -461,14 +461,21 @@ void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, h
 template <bool SYNC, bool MG>
 static void launch_moves(const DevGraph &g, const DevState &s, const ClassLists &cl, const LoopParams &p,
                          hipStream_t st) {
-  if (cl.c1 > 0)
-    k_move_lane<SYNC, MG><<<grid_for(cl.c1, kBlock, kGridCap), kBlock, 0, st>>>(g, s, cl.list, cl.c1, p.pickstep);
-  if (cl.c2 > cl.c1)
-    k_move_group<SYNC, MG, 64><<<grid_for(cl.c2 - cl.c1, kBlock / 64, kGridCap), kBlock, 0, st>>>(
-        g, s, cl.list + cl.c1, cl.c2 - cl.c1, p.pickstep);
-  if (cl.n > cl.c2)
-    k_move_group<SYNC, MG, 256><<<grid_for(cl.n - cl.c2, 1, kGridCap), kBlock, 0, st>>>(
-        g, s, cl.list + cl.c2, cl.n - cl.c2, p.pickstep);
+  const int S = cl.sweeps;
+  for (int w = 0; w < S; ++w) {
+    const uint64_t b0 = cl.seg[w], e0 = cl.seg[w + 1];
+    const uint64_t b1 = cl.seg[S + w], e1 = cl.seg[S + w + 1];
+    const uint64_t b2 = cl.seg[2 * S + w], e2 = cl.seg[2 * S + w + 1];
+    if (e0 > b0)
+      k_move_lane<SYNC, MG><<<grid_for(e0 - b0, kBlock, kGridCap), kBlock, 0, st>>>(g, s, cl.list + b0, e0 - b0,
+                                                                                    p.pickstep);
+    if (e1 > b1)
+      k_move_group<SYNC, MG, 64><<<grid_for(e1 - b1, kBlock / 64, kGridCap), kBlock, 0, st>>>(
+          g, s, cl.list + b1, e1 - b1, p.pickstep);
+    if (e2 > b2)
+      k_move_group<SYNC, MG, 256><<<grid_for(e2 - b2, 1, kGridCap), kBlock, 0, st>>>(g, s, cl.list + b2, e2 - b2,
+                                                                                     p.pickstep);
+  }
 }
 
 void launch_iteration(const DevGraph &g, const DevState &s, const ClassLists &cl, const LoopParams &p,

@@ -34,11 +34,13 @@ struct DevState {
   DevCounters *ctr;
 };
 
-// Degree classes: list[0 .. c1) lane-per-vertex, [c1 .. c2) wave-per-vertex,
-// [c2 .. n) workgroup-per-vertex; ascending vertex id inside each class.
+// Vertex order of an iteration: list[seg[c * sweeps + s] .. seg[c * sweeps + s + 1]) holds the
+// vertices of degree class c (0 lane-per-vertex, 1 wave-per-vertex, 2 workgroup-per-vertex) in
+// sub-sweep s. Sub-sweeps run one after the other; inside one, the three classes run in turn.
 struct ClassLists {
   const uint32_t *list;
-  uint64_t c1, c2, n;
+  const uint64_t *seg;  // host array of 3 * sweeps + 1 entries
+  int sweeps;
 };
 
 struct LoopParams {
