@@ -43,8 +43,9 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--wave-degree", type=int, default=0)
-    ap.add_argument("--block-degree", type=int, default=0)
+    ap.add_argument("--group-degree", type=str, default="0,0,0,0",
+                    help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
+    ap.add_argument("--sweeps", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -60,7 +61,7 @@ def main():
     log(f"graph: R-MAT scale {args.scale}: n={g.n} m={g.m} max_degree={int(g.degrees().max())} "
         f"({time.time() - t0:.1f}s to generate)")
     opts = rlc.options(mode=rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
-                       wave_degree=args.wave_degree, block_degree=args.block_degree)
+                       group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps)
     ctx = rlc.Context(0)
     dg = ctx.upload(g)
     for _ in range(args.warmup):

@@ -75,15 +75,17 @@ typedef struct rlc_options {
                              convergence test; 0 disables [8] */
   int32_t merge_degree;   /* SYNC: vertices of degree >= merge_degree use the P-part merge-tree sketch,
                              0 = never (every vertex scans its edges sequentially) [0] */
-  int32_t merge_parts;    /* SYNC: P, 64 or 256 [64] */
-  int32_t wave_degree;    /* ASYNC: degree >= wave_degree -> one wavefront per vertex (0 = library default) */
-  int32_t block_degree;   /* ASYNC: degree >= block_degree -> one workgroup per vertex (0 = library default) */
+  int32_t merge_parts;    /* SYNC: P, one of 4, 16, 64, 256 [64] */
+  int32_t group_degree[4];/* ASYNC: vertices of degree >= group_degree[k] are scanned by 4, 16, 64 (one
+                             wavefront) or 256 (one workgroup) lanes for k = 0..3; below
+                             group_degree[0] one lane scans a vertex. Non-decreasing; all zero = the
+                             library defaults {32, 128, 512, 4096} */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
-  int32_t sweeps;         /* ASYNC: each iteration visits the vertices in S sub-sweeps, one launch
-                             group each, vertex u in sub-sweep (u / 64) % S, so that later
-                             sub-sweeps see the labels written by earlier ones (0 = library
-                             default: up to 8, at least 4096 vertices per sub-sweep). SYNC: ignored */
-  int32_t reserved[6];    /* must be zero */
+  int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: vertices are
+                             split by (id / 64) % S into S segments, run one after the other so that
+                             later segments see the labels written by earlier ones); 1 = a single
+                             sweep. SYNC: ignored */
+  int32_t reserved[4];    /* must be zero */
 } rlc_options;
 
 typedef struct rlc_run_stats {
@@ -112,6 +114,8 @@ typedef struct rlc_graph_info {
   uint64_t device_bytes; /* graph arrays + per-graph state */
   int32_t unit_weights;  /* weights were NULL (no weight array stored on the device) */
   int32_t offsets_bytes; /* 8 */
+  int32_t segments;      /* sub-sweep segments of the asynchronous mode */
+  int32_t reserved;
 } rlc_graph_info;
 
 /* ---- errors -------------------------------------------------------------------------------- */
@@ -147,8 +151,14 @@ void rlc_graph_destroy(rlc_graph *g);
  * resident on the device (rlc_graph_labels_device, rlc_modularity with labels == NULL). */
 rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, const rlc_options *o,
                    uint32_t *out_labels, rlc_run_stats *stats);
-/* Device pointer of the resident labels of g (valid until the next run or destroy). */
+/* Device pointer of the resident labels of g (valid until the next run or destroy). Entry p is
+ * the label of the vertex at position p of the library's vertex order (see below); label values
+ * are caller vertex ids (or the caller's initial label values). */
 const uint32_t *rlc_graph_labels_device(const rlc_graph *g);
+/* The library stores a graph in degree-descending vertex order: order[p] = caller id of the
+ * vertex at position p (n entries). Host arrays passed to or returned by rlc_run and
+ * rlc_modularity are always in caller order. */
+rlc_status rlc_graph_vertex_order(const rlc_graph *g, uint32_t *order);
 
 /* FP64 modularity on the device, Q = sum_c cin_c/(2M) - (tot_c/(2M))^2 with 2M = sum of all
  * weights, and the number of distinct labels. labels: host array of n entries, or NULL to use

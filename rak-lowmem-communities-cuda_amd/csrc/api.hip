@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstring>
 #include <mutex>
@@ -31,21 +32,26 @@ struct rlc_context {
 
 namespace {
 
-constexpr uint64_t kNoLimit = ~0ull;
-constexpr int kDefaultWaveDegree = 32;
-constexpr int kDefaultBlockDegree = 2048;
-constexpr int kDefaultSweeps = 8;
-constexpr uint64_t kMinSweepVertices = 4096;
+constexpr int kDefaultGroupDegree[4] = {32, 128, 512, 4096};  // 4, 16, 64, 256 lanes per vertex
+constexpr int kMaxSegments = 8;
+constexpr uint64_t kSegmentVertices = 4096;  // at least this many vertices per segment
 
+int segments_for(uint64_t n) {
+  const uint64_t s = n / kSegmentVertices;
+  return (int)std::max<uint64_t>(1, std::min<uint64_t>(kMaxSegments, s));
+}
+
+// Everything a captured iteration loop depends on besides the graph itself.
 struct ExecKey {
   bool valid = false;
-  int sync, mg, pickstep, max_iterations;
+  int sync, mg, pickstep, max_iterations, merge_degree, merge_parts, sweeps;
+  int group_degree[4];
   double tolerance;
-  uint64_t t1, t2;
-  int sweeps;
   bool operator==(const ExecKey &o) const {
     return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
-           max_iterations == o.max_iterations && tolerance == o.tolerance && t1 == o.t1 && t2 == o.t2 && sweeps == o.sweeps;
+           max_iterations == o.max_iterations && merge_degree == o.merge_degree && merge_parts == o.merge_parts &&
+           sweeps == o.sweeps && tolerance == o.tolerance &&
+           std::equal(group_degree, group_degree + 4, o.group_degree);
   }
 };
 
@@ -67,16 +73,14 @@ struct rlc_graph {
   rlc_context *ctx = nullptr;
   uint64_t n = 0, m = 0, max_degree = 0;
   bool unit = false;
+  // degree-descending layout: h_off are the layout's offsets, h_perm[p] = original id at p
   std::vector<uint64_t> h_off;
+  std::vector<uint32_t> h_perm;
+  std::vector<uint64_t> h_seg;  // first position of each sub-sweep segment
   uint64_t *d_off = nullptr;
   uint32_t *d_edg = nullptr;
   float *d_wei = nullptr;
-  // degree classes
-  uint32_t *d_list = nullptr;
-  uint64_t list_t1 = 0, list_t2 = 0;
-  int list_sweeps = 0;
-  std::vector<uint64_t> seg;
-  bool list_valid = false;
+  uint32_t *d_perm = nullptr;
   // state
   uint32_t *d_lab = nullptr;
   uint8_t *d_aff = nullptr;
@@ -117,7 +121,7 @@ void free_graph(rlc_graph *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->hexec) (void)hipGraphExecDestroy(g->hexec);
   if (g->hgraph) (void)hipGraphDestroy(g->hgraph);
-  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_list, g->d_lab, g->d_aff, g->d_moves,
+  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_aff, g->d_moves,
                   g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -129,24 +133,78 @@ void free_graph(rlc_graph *g) {
   delete g;
 }
 
-rlc_status ensure_lists(rlc_graph *g, uint64_t t1, uint64_t t2, int sweeps) {
-  if (g->list_valid && g->list_t1 == t1 && g->list_t2 == t2 && g->list_sweeps == sweeps) return RLC_OK;
-  std::vector<uint32_t> list;
-  std::vector<uint64_t> seg;
-  build_class_lists(g->n, g->h_off.data(), t1, t2, sweeps, list, seg);
-  if (!g->d_list) {
-    rlc_status st = dev_alloc(&g->d_list, g->n);
-    if (st != RLC_OK) return st;
+// First position in [lo, hi) of segment-sorted (degree-descending) positions whose degree is < t.
+uint64_t first_below(const rlc_graph *g, uint64_t lo, uint64_t hi, uint64_t t) {
+  while (lo < hi) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (g->h_off[mid + 1] - g->h_off[mid] < t) hi = mid;
+    else lo = mid + 1;
   }
-  HIP_TRY(hipMemcpyAsync(g->d_list, list.data(), g->n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  g->list_t1 = t1;
-  g->list_t2 = t2;
-  g->list_sweeps = sweeps;
-  g->seg.swap(seg);
-  g->list_valid = true;
-  g->key.valid = false;  // captured launches embed the class sizes
-  return RLC_OK;
+  return lo;
+}
+
+void add_range(RangeSet &rs, uint64_t first, uint64_t last, bool chunks) {
+  if (last <= first) return;
+  const uint32_t r = rs.count++;
+  rs.first[r] = first;
+  rs.last[r] = last;
+  rs.acc[r + 1] = rs.acc[r] + (chunks ? (last - first + 63) / 64 : last - first);
+}
+
+RangeSet empty_ranges() {
+  RangeSet rs;
+  std::memset(&rs, 0, sizeof(rs));
+  return rs;
+}
+
+// The move launches of one iteration. Each segment of the layout is degree-descending, so every
+// degree class is a contiguous range of it.
+//   SYNC  (order-free): one launch of P-lane groups over the prefixes with degree >= merge_degree,
+//         one lane launch over the rest.
+//   ASYNC: hubs (256 lanes) of every segment first, then segment after segment its 64-, 16-, 4-lane
+//         classes and its lane class (or all segments in one launch per class when sweeps == 1).
+Schedule build_schedule(const rlc_graph *g, const rlc_options *o) {
+  Schedule sc;
+  const size_t S = g->h_seg.size() - 1;
+  if (o->mode == RLC_MODE_SYNC) {
+    const uint64_t D = o->merge_degree > 0 ? (uint64_t)o->merge_degree : ~0ull;
+    MoveLaunch grp{o->merge_parts, empty_ranges()}, lane{1, empty_ranges()};
+    for (size_t s = 0; s < S; ++s) {
+      const uint64_t sb = g->h_seg[s], se = g->h_seg[s + 1];
+      const uint64_t cut = D == ~0ull ? sb : first_below(g, sb, se, D);
+      add_range(grp.ranges, sb, cut, false);
+      add_range(lane.ranges, cut, se, true);
+    }
+    sc.launches.push_back(grp);
+    sc.launches.push_back(lane);
+    return sc;
+  }
+  uint64_t t[4];
+  const bool dflt = o->group_degree[0] == 0 && o->group_degree[1] == 0 && o->group_degree[2] == 0 &&
+                    o->group_degree[3] == 0;
+  for (int k = 0; k < 4; ++k) t[k] = (uint64_t)(dflt ? kDefaultGroupDegree[k] : o->group_degree[k]);
+  static const int kLanes[5] = {256, 64, 16, 4, 1};
+  // per segment, the class cut points: [sb, c0) >= t3, [c0, c1) >= t2, [c1, c2) >= t1, [c2, c3) >= t0, [c3, se)
+  std::vector<std::array<uint64_t, 6>> cuts(S);
+  for (size_t s = 0; s < S; ++s) {
+    const uint64_t sb = g->h_seg[s], se = g->h_seg[s + 1];
+    cuts[s][0] = sb;
+    for (int k = 0; k < 4; ++k) cuts[s][k + 1] = first_below(g, sb, se, t[3 - k]);
+    cuts[s][5] = se;
+  }
+  MoveLaunch hubs{256, empty_ranges()};
+  for (size_t s = 0; s < S; ++s) add_range(hubs.ranges, cuts[s][0], cuts[s][1], false);
+  sc.launches.push_back(hubs);
+  const bool one_sweep = o->sweeps == 1;
+  for (size_t s = 0; s < (one_sweep ? 1 : S); ++s) {
+    for (int c = 1; c < 5; ++c) {
+      MoveLaunch L{kLanes[c], empty_ranges()};
+      for (size_t q = one_sweep ? 0 : s; q < (one_sweep ? S : s + 1); ++q)
+        add_range(L.ranges, cuts[q][c], cuts[q][c + 1], kLanes[c] == 1);
+      sc.launches.push_back(L);
+    }
+  }
+  return sc;
 }
 
 rlc_status check_options(const rlc_options *o) {
@@ -160,38 +218,21 @@ rlc_status check_options(const rlc_options *o) {
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.mode must be RLC_MODE_ASYNC or RLC_MODE_SYNC");
   if (o->pickstep < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.pickstep must be >= 0");
   if (o->merge_degree < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_degree must be >= 0");
-  if (o->mode == RLC_MODE_SYNC && o->merge_degree > 0 && o->merge_parts != 64 && o->merge_parts != 256)
-    return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts must be 64 or 256");
-  if (o->wave_degree < 0 || o->block_degree < 0)
-    return fail(RLC_ERR_INVALID_ARGUMENT, "options.wave_degree / block_degree must be >= 0");
-  if (o->sweeps < 0 || o->sweeps > RLC_MAX_SWEEPS)
-    return fail(RLC_ERR_INVALID_ARGUMENT, "options.sweeps must be in [0, RLC_MAX_SWEEPS]");
+  if (o->mode == RLC_MODE_SYNC && o->merge_degree > 0 && o->merge_parts != 4 && o->merge_parts != 16 &&
+      o->merge_parts != 64 && o->merge_parts != 256)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts must be 4, 16, 64 or 256");
+  for (int k = 0; k < 4; ++k) {
+    if (o->group_degree[k] < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.group_degree must be >= 0");
+    if (k && o->group_degree[k] < o->group_degree[k - 1])
+      return fail(RLC_ERR_INVALID_ARGUMENT, "options.group_degree must be non-decreasing");
+  }
+  if (o->sweeps != 0 && o->sweeps != 1)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.sweeps must be 0 (the graph's segments) or 1");
   if (o->timing != RLC_TIMING_GRAPH && o->timing != RLC_TIMING_PER_ITERATION)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.timing must be RLC_TIMING_GRAPH or RLC_TIMING_PER_ITERATION");
   for (int r : o->reserved)
     if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.reserved must be zero");
   return RLC_OK;
-}
-
-// sub-sweeps of an asynchronous iteration (the synchronous mode reads a snapshot: one sweep)
-int sweep_count(const rlc_options *o, uint64_t n) {
-  if (o->mode == RLC_MODE_SYNC) return 1;
-  if (o->sweeps > 0) return o->sweeps;
-  uint64_t s = n / kMinSweepVertices;
-  return (int)std::max<uint64_t>(1, std::min<uint64_t>(kDefaultSweeps, s));
-}
-
-// degree thresholds (t1, t2) of the lane / wave / workgroup classes
-void class_thresholds(const rlc_options *o, uint64_t &t1, uint64_t &t2) {
-  if (o->mode == RLC_MODE_SYNC) {
-    if (o->merge_degree <= 0) { t1 = t2 = kNoLimit; return; }
-    t1 = (uint64_t)o->merge_degree;
-    t2 = o->merge_parts == 64 ? kNoLimit : t1;
-    return;
-  }
-  t1 = (uint64_t)(o->wave_degree > 0 ? o->wave_degree : kDefaultWaveDegree);
-  t2 = (uint64_t)(o->block_degree > 0 ? o->block_degree : kDefaultBlockDegree);
-  if (t2 < t1) t2 = t1;
 }
 
 rlc_status check_launch() {
@@ -270,6 +311,9 @@ static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::v
   uint64_t md = 0;
   rlc_status st = validate_csr(n, m, off.data(), edges, weights, &md);
   if (st != RLC_OK) return st;
+  Layout L;
+  if ((st = build_layout(n, m, off.data(), edges, weights, segments_for(n), L)) != RLC_OK) return st;
+  std::vector<uint64_t>().swap(off);
   HIP_TRY(hipSetDevice(ctx->device));
   rlc_graph *g = new rlc_graph();
   g->ctx = ctx;
@@ -277,7 +321,6 @@ static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::v
   g->m = m;
   g->max_degree = md;
   g->unit = weights == nullptr;
-  g->h_off = std::move(off);
   auto bail = [&](rlc_status s) { free_graph(g); return s; };
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "hipStreamCreate failed"));
@@ -288,17 +331,22 @@ static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::v
   if ((st = dev_alloc(&g->d_off, n + 1)) != RLC_OK) return bail(st);
   if ((st = dev_alloc(&g->d_edg, m)) != RLC_OK) return bail(st);
   if (!g->unit && (st = dev_alloc(&g->d_wei, m)) != RLC_OK) return bail(st);
+  if ((st = dev_alloc(&g->d_perm, n)) != RLC_OK) return bail(st);
   if ((st = dev_alloc(&g->d_lab, n)) != RLC_OK) return bail(st);
   if ((st = dev_alloc(&g->d_aff, n)) != RLC_OK) return bail(st);
   if ((st = dev_alloc(&g->d_ctr, 1)) != RLC_OK) return bail(st);
   auto cp = [&](void *dst, const void *src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->stream) : hipSuccess;
   };
-  if (cp(g->d_off, g->h_off.data(), (n + 1) * sizeof(uint64_t)) != hipSuccess ||
-      cp(g->d_edg, edges, m * sizeof(uint32_t)) != hipSuccess ||
-      (!g->unit && cp(g->d_wei, weights, m * sizeof(float)) != hipSuccess) ||
+  if (cp(g->d_off, L.off.data(), (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+      cp(g->d_edg, L.edg.data(), m * sizeof(uint32_t)) != hipSuccess ||
+      (!g->unit && cp(g->d_wei, L.wei.data(), m * sizeof(float)) != hipSuccess) ||
+      cp(g->d_perm, L.perm.data(), n * sizeof(uint32_t)) != hipSuccess ||
       hipStreamSynchronize(g->stream) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "graph upload copy failed"));
+  g->h_off.swap(L.off);
+  g->h_perm.swap(L.perm);
+  g->h_seg.swap(L.seg);
   *out = g;
   return RLC_OK;
 }
@@ -331,7 +379,9 @@ rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info) {
   info->max_degree = g->max_degree;
   info->unit_weights = g->unit ? 1 : 0;
   info->offsets_bytes = 8;
-  info->device_bytes = (g->n + 1) * 8 + g->m * 4 + (g->unit ? 0 : g->m * 4) + g->n * (4 + 1 + 4) +
+  info->segments = (int32_t)(g->h_seg.size() - 1);
+  info->reserved = 0;
+  info->device_bytes = (g->n + 1) * 8 + g->m * 4 + (g->unit ? 0 : g->m * 4) + g->n * (4 + 4 + 1) +
                        (g->d_moves ? g->n * 8 : 0) + sizeof(DevCounters);
   return RLC_OK;
 }
@@ -339,6 +389,13 @@ rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info) {
 void rlc_graph_destroy(rlc_graph *g) { free_graph(g); }
 
 const uint32_t *rlc_graph_labels_device(const rlc_graph *g) { return g ? g->d_lab : nullptr; }
+
+rlc_status rlc_graph_vertex_order(const rlc_graph *g, uint32_t *order) {
+  clear_error();
+  if (!g || (!order && g->n)) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_vertex_order: null argument");
+  std::copy(g->h_perm.begin(), g->h_perm.end(), order);
+  return RLC_OK;
+}
 
 rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, const rlc_options *o,
                    uint32_t *out_labels, rlc_run_stats *stats) {
@@ -357,19 +414,18 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
       dense &= init_labels[u] < n;
     }
     if (!g->d_init && (st = dev_alloc(&g->d_init, n)) != RLC_OK) return st;
-    HIP_TRY(hipMemcpyAsync(g->d_init, init_labels, n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+    std::vector<uint32_t> ip(n);  // to layout order
+    for (uint64_t p = 0; p < n; ++p) ip[p] = init_labels[g->h_perm[p]];
+    HIP_TRY(hipMemcpyAsync(g->d_init, ip.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
   }
   const bool sync = o->mode == RLC_MODE_SYNC;
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
-  uint64_t t1, t2;
-  class_thresholds(o, t1, t2);
-  const int sweeps = sweep_count(o, n);
-  if ((st = ensure_lists(g, t1, t2, sweeps)) != RLC_OK) return st;
+  const Schedule cl = build_schedule(g, o);
 
   const DevGraph dg = g->dev();
   const DevState ds = g->state();
-  const ClassLists cl{g->d_list, g->seg.data(), sweeps};
-  LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, o->pickstep, o->tolerance, o->max_iterations, 256};
+  LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, o->pickstep, o->tolerance, o->max_iterations};
   const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
 
   if (graph_mode && o->max_iterations > 0) {
@@ -380,9 +436,10 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     k.pickstep = o->pickstep;
     k.max_iterations = o->max_iterations;
     k.tolerance = o->tolerance;
-    k.t1 = t1;
-    k.t2 = t2;
-    k.sweeps = sweeps;
+    k.merge_degree = sync ? o->merge_degree : 0;
+    k.merge_parts = sync ? o->merge_parts : 0;
+    k.sweeps = sync ? 0 : o->sweeps;
+    for (int q = 0; q < 4; ++q) k.group_degree[q] = sync ? 0 : o->group_degree[q];
     if (!(k == g->key)) {
       if (g->hexec) { (void)hipGraphExecDestroy(g->hexec); g->hexec = nullptr; }
       if (g->hgraph) { (void)hipGraphDestroy(g->hgraph); g->hgraph = nullptr; }
@@ -403,7 +460,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   for (int r = 0; r < o->repeat; ++r) {
     iter_ms.clear();
     HIP_TRY(hipEventRecord(g->ev[0], g->stream));
-    launch_init_labels(dg, ds, init_labels ? g->d_init : nullptr, g->stream);
+    launch_init_labels(dg, ds, init_labels ? g->d_init : nullptr, g->d_perm, g->stream);
     HIP_TRY(hipEventRecord(g->ev[1], g->stream));
     launch_mark_all(dg, ds, o->max_iterations, g->stream);
     HIP_TRY(hipEventRecord(g->ev[2], g->stream));
@@ -438,7 +495,11 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
 
   DevCounters hc;
   HIP_TRY(hipMemcpy(&hc, g->d_ctr, sizeof(hc), hipMemcpyDeviceToHost));
-  if (out_labels && n) HIP_TRY(hipMemcpy(out_labels, g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if (out_labels && n) {
+    std::vector<uint32_t> lp(n);
+    HIP_TRY(hipMemcpy(lp.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint64_t p = 0; p < n; ++p) out_labels[g->h_perm[p]] = lp[p];  // back to caller order
+  }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     stats->iterations = (int32_t)hc.iter;
@@ -484,9 +545,12 @@ rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels
   std::vector<uint32_t> dense;
   if (labels || !g->labels_dense) {
     // labels must index the per-community accumulators: map values >= n to dense ids
-    std::vector<uint32_t> src(n);
-    if (labels) std::copy(labels, labels + n, src.begin());
-    else if (n) HIP_TRY(hipMemcpy(src.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> src(n);  // labels in layout order
+    if (labels) {
+      for (uint64_t p = 0; p < n; ++p) src[p] = labels[g->h_perm[p]];
+    } else if (n) {
+      HIP_TRY(hipMemcpy(src.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    }
     bool ok = true;
     for (uint64_t u = 0; u < n; ++u) ok &= src[u] < n;
     if (!ok) {

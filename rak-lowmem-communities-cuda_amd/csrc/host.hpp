@@ -12,10 +12,20 @@ const char *last_error_cstr();
 rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
                         uint64_t *max_degree);
 
-// Vertex order of one iteration: grouped by degree class (0: deg < t1, 1: t1 <= deg < t2,
-// 2: deg >= t2), then by sub-sweep (u / 64) % sweeps, ascending id inside each group.
-// seg[c * sweeps + s] is the first position of (class c, sub-sweep s); seg[3 * sweeps] = n.
-void build_class_lists(uint64_t n, const uint64_t *off, uint64_t t1, uint64_t t2, int sweeps,
-                       std::vector<uint32_t> &list, std::vector<uint64_t> &seg);
+// Device layout of a graph: vertices split into `segments` sub-sweep segments by
+// (id / 64) % segments, stored segment after segment, by degree descending inside a segment
+// (ties: ascending id), so that the most referenced labels share cache lines and every degree
+// class is a contiguous range of each segment. perm[p] = original id of the vertex at position
+// p, inv[u] = position of original vertex u, seg[s] = first position of segment s (seg[S] = n).
+// Each row keeps its original edge order; targets are renumbered to positions.
+struct Layout {
+  std::vector<uint32_t> perm, inv;
+  std::vector<uint64_t> seg;
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> edg;
+  std::vector<float> wei;  // empty for unit weights
+};
+rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
+                        int segments, Layout &out);
 
 }  // namespace rlc

@@ -55,10 +55,93 @@ __device__ __forceinline__ void flush_stats(DevCounters *ctr, unsigned long long
 
 __device__ __forceinline__ float edge_w(const DevGraph &g, uint64_t i) { return g.wei ? g.wei[i] : 1.0f; }
 
+constexpr int kBatch = 8;  // edges in flight per lane
+
+// Visit adjacency positions i, i+STRIDE, i+2*STRIDE, .. < e of vertex u in order, calling
+// f(label[v], w) for every target v != u. Positions are fetched kBatch at a time (targets and
+// weights, then the label gathers) so that each lane keeps kBatch independent loads in flight;
+// full batches are unpredicated, one predicated batch takes the tail. Indices are 32-bit
+// relative to the first position (a degree is below 2^32).
+template <bool UNIT, int STRIDE, class F>
+__device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t i, uint64_t e,
+                                          F f) {
+  if (i >= e) return;
+  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
+  const uint32_t *ep = g.edg + i;
+  const float *wp = UNIT ? nullptr : g.wei + i;
+  uint32_t j = 0;
+  for (; j + kBatch <= cnt; j += kBatch) {
+    uint32_t vv[kBatch], cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      vv[k] = ep[(j + k) * STRIDE];
+      ww[k] = UNIT ? 1.0f : wp[(j + k) * STRIDE];
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));  // all loads issued before use
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (vv[k] != u) f(cc[k], ww[k]);
+  }
+  if (j < cnt) {
+    const uint32_t rem = cnt - j;
+    uint32_t vv[kBatch], cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint32_t jj = j + ((uint32_t)k < rem ? (uint32_t)k : 0u);
+      vv[k] = ep[jj * STRIDE];
+      ww[k] = UNIT ? 1.0f : wp[jj * STRIDE];
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if ((uint32_t)k < rem && vv[k] != u) f(cc[k], ww[k]);
+  }
+}
+
+// aff[v] = 1 for targets v != u at positions i, i+stride, .. < e. The flag is read first and
+// written only when clear: the flags of the most connected vertices share a few cache lines
+// (degree-descending layout), and unconditional stores from every moved neighbour would
+// serialise on them.
+__device__ __forceinline__ void mark_neighbours(const DevGraph &g, uint8_t *aff, uint32_t u, uint64_t i, uint64_t e,
+                                                uint64_t stride) {
+  for (; i < e; i += stride * kBatch) {
+    uint32_t vv[kBatch];
+    uint8_t fl[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint64_t j = i + (uint64_t)k * stride;
+      vv[k] = j < e ? g.edg[j] : u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) fl[k] = aff[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (vv[k] != u && fl[k] == 0) aff[vv[k]] = 1;
+  }
+}
+
+__device__ __forceinline__ uint32_t range_of(const RangeSet &rs, uint64_t i) {
+  uint32_t lo = 0, hi = rs.count;  // acc[lo] <= i < acc[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rs.acc[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // ---- lane per vertex: one sequential sketch per lane ----------------------------------------
-template <bool SYNC, bool MG>
-__global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, const uint32_t *__restrict__ list,
-                                                      uint64_t count, int pickstep) {
+// Items are 64-position chunks of the ranges of `rs`; one chunk per wave.
+template <bool SYNC, bool MG, bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, RangeSet rs, int pickstep) {
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
   const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
@@ -66,10 +149,18 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, co
   __shared__ unsigned int wcount[kBlock / 64];
   __shared__ unsigned long long bbase;
   const int lane = lane_id(), wave = threadIdx.x >> 6;
-  for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < count; base += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t idx = base + threadIdx.x;
-    const bool valid = idx < count;
-    const uint32_t u = valid ? list[idx] : 0u;
+  constexpr uint64_t WPB = kBlock / 64;
+  const uint64_t nch = rs.acc[rs.count];
+  for (uint64_t tb = (uint64_t)blockIdx.x * WPB; tb < nch; tb += (uint64_t)gridDim.x * WPB) {
+    const uint64_t t = tb + (uint64_t)wave;
+    bool valid = false;
+    uint32_t u = 0;
+    if (t < nch) {
+      const uint32_t r = range_of(rs, t);
+      const uint64_t p = rs.first[r] + (t - rs.acc[r]) * 64 + (uint64_t)lane;
+      valid = p < rs.last[r];
+      u = valid ? (uint32_t)p : 0u;
+    }
     const bool act = valid && s.aff[u] != 0;
     uint64_t deg = 0;
     bool moved = false;
@@ -82,31 +173,20 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, co
       if (MG) {
         MG8 sk;
         sk.clear();
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          sk.accumulate(s.lab[v], edge_w(g, i));
-        }
-        float acc[kSlots];
+        for_edges<UNIT, 1>(g, s.lab, u, b, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
+        if (sk.dec) {  // rescan (else the slot weights are already exact)
 #pragma unroll
-        for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          const uint32_t c = s.lab[v];
-          const float w = edge_w(g, i);
+          for (int j = 0; j < kSlots; ++j) sk.w[j] = 0.0f;
+          for_edges<UNIT, 1>(g, s.lab, u, b, e, [&](uint32_t c, float w) {
 #pragma unroll
-          for (int j = 0; j < kSlots; ++j) acc[j] = sk.c[j] == c ? acc[j] + w : acc[j];
+            for (int j = 0; j < kSlots; ++j) sk.w[j] = sk.c[j] == c ? sk.w[j] + w : sk.w[j];
+          });
         }
-        argmax8(sk.c, acc, cb, wb);
+        argmax8(sk.c, sk.w, cb, wb);
       } else {
         uint32_t mc = kEmpty;
         float mw = 0.0f;
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          bm_step(mc, mw, s.lab[v], edge_w(g, i));
-        }
+        for_edges<UNIT, 1>(g, s.lab, u, b, e, [&](uint32_t c, float w) { bm_step(mc, mw, c, w); });
         cb = mc;
         wb = mw;
       }
@@ -114,10 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, co
       s.aff[u] = 0;
       if (!SYNC && moved) {
         s.lab[u] = cb;
-        for (uint64_t i = b; i < e; ++i) {
-          const uint32_t v = g.edg[i];
-          if (v != u) s.aff[v] = 1;
-        }
+        mark_neighbours(g, s.aff, u, b, e, 1);
       }
     }
     sa += act ? 1 : 0;
@@ -128,9 +205,9 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, co
       if (lane == 0) wcount[wave] = (unsigned int)__popcll(mask);
       __syncthreads();
       if (threadIdx.x == 0) {
-        unsigned int t = 0;
-        for (int w = 0; w < kBlock / 64; ++w) t += wcount[w];
-        bbase = t ? atomicAdd(&s.ctr->nmoves, (unsigned long long)t) : 0ull;
+        unsigned int tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wcount[w];
+        bbase = tot ? atomicAdd(&s.ctr->nmoves, (unsigned long long)tot) : 0ull;
       }
       __syncthreads();
       if (moved) {
@@ -148,64 +225,83 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, co
   flush_stats(s.ctr, sa, se, sc, sx);
 }
 
-// ---- P lanes per vertex (P = 64: one wave; P = 256: four waves of one workgroup) -----------
-template <bool SYNC, bool MG, int P>
-__global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, const uint32_t *__restrict__ list,
-                                                       uint64_t count, int pickstep) {
-  constexpr int GW = P / 64;     // waves per vertex
-  constexpr int VPB = kBlock / P;  // vertices per block round
+// ---- P lanes per vertex: groups of P = 4, 16, 64 lanes inside a wave, or P = 256 (4 waves) --
+// Items are vertices of the ranges of `rs`; part p of a vertex scans positions p, p+P, ...
+template <bool SYNC, bool MG, bool UNIT, int P>
+__global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, RangeSet rs, int pickstep) {
+  constexpr int W = P < 64 ? P : 64;  // lanes of a group inside one wave
+  constexpr int GW = P / W;           // waves per vertex
+  constexpr int VPB = kBlock / P;     // vertices per block round
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
   const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
-  const int lane = lane_id(), wave = threadIdx.x >> 6;
-  const int grp = wave / GW;  // vertex slot in the block
-  const int gw = wave % GW;   // wave inside the group
-  const int gl = (int)threadIdx.x - grp * P;  // part index 0..P-1
+  const int lane = lane_id();
+  const int grp = (int)threadIdx.x / P;  // vertex slot in the block
+  const int gl = (int)threadIdx.x % P;   // part index
+  const int gw = gl / 64;                // wave inside the vertex (P = 256)
   __shared__ uint32_t sh_c[GW][kSlots];
   __shared__ float sh_w[GW][kSlots];
   __shared__ uint32_t sh_cand[kSlots];
   __shared__ float sh_tot[kSlots];
-  __shared__ uint32_t sh_flag;
+  __shared__ uint32_t sh_flag, sh_flag2;
+  __shared__ uint32_t sh_dec[GW];
   unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
-  for (uint64_t base = (uint64_t)blockIdx.x * VPB; base < count; base += (uint64_t)gridDim.x * VPB) {
-    const uint64_t idx = base + grp;
-    const bool valid = idx < count;
-    const uint32_t u = valid ? list[idx] : 0u;
+  const uint64_t total = rs.acc[rs.count];
+  for (uint64_t base = (uint64_t)blockIdx.x * VPB; base < total; base += (uint64_t)gridDim.x * VPB) {
+    const uint64_t idx = base + (uint64_t)grp;
+    const bool valid = idx < total;
+    uint32_t u = 0;
+    if (valid) {
+      const uint32_t r = range_of(rs, idx);
+      u = (uint32_t)(rs.first[r] + (idx - rs.acc[r]));
+    }
     bool act;
     if (GW == 1) {
-      act = valid && uniform_u32(s.aff[u]) != 0;
+      act = valid && s.aff[u] != 0;  // one load instruction for the whole group
     } else {
       if (threadIdx.x == 0) sh_flag = valid ? s.aff[u] : 0u;
       __syncthreads();
       act = sh_flag != 0;
     }
+    uint64_t deg = 0;
+    bool moved = false;
+    uint32_t cb = 0;
     if (act) {
       const uint64_t b = g.off[u], e = g.off[(uint64_t)u + 1];
-      const uint64_t deg = e - b;
+      deg = e - b;
       const uint32_t d = s.lab[u];
-      uint32_t cb;
-      float wb;
+      float wb = 0.0f;
       if (MG) {
         MG8 sk;
         sk.clear();
-        for (uint64_t i = b + gl; i < e; i += P) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          sk.accumulate(s.lab[v], edge_w(g, i));
-        }
-        wave_merge_tree(sk);
+        for_edges<UNIT, P>(g, s.lab, u, b + gl, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
+        group_merge_tree<W>(sk);
         uint32_t cand[kSlots];
+        float cw[kSlots];  // merged weights: exact when no decrement happened in the group
+        bool any_dec;
         if (GW == 1) {
+          const int leader = lane & ~(W - 1);
 #pragma unroll
-          for (int j = 0; j < kSlots; ++j) cand[j] = __shfl(sk.c[j], 0);
+          for (int j = 0; j < kSlots; ++j) {
+            cand[j] = __shfl(sk.c[j], leader);
+            cw[j] = __shfl(sk.w[j], leader);
+          }
+          // any lane of the group that decremented, in the scan or in an absorb
+          const unsigned long long dm = __ballot(sk.dec);
+          const unsigned long long gm = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (lane & ~(W - 1) & 63));
+          any_dec = (dm & gm) != 0ull;
         } else {
+          const bool wdec = __ballot(sk.dec) != 0ull;
           if (lane == 0) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) { sh_c[gw][j] = sk.c[j]; sh_w[gw][j] = sk.w[j]; }
+            sh_dec[gw] = wdec ? 1u : 0u;
           }
           __syncthreads();
           if (threadIdx.x == 0) {
             // continue the binary tree across the waves: wave q absorbs wave q+k
+            bool d = false;
+            for (int q = 0; q < GW; ++q) d = d || sh_dec[q] != 0u;
             for (int k = 1; k < GW; k <<= 1)
               for (int q = 0; q + k < GW; q += 2 * k) {
                 MG8 a;
@@ -215,27 +311,32 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, c
                   a.c[j] = sh_c[q][j]; a.w[j] = sh_w[q][j];
                   oc[j] = sh_c[q + k][j]; ow[j] = sh_w[q + k][j];
                 }
+                a.dec = false;
                 a.absorb(oc, ow);
+                d = d || a.dec;
                 for (int j = 0; j < kSlots; ++j) { sh_c[q][j] = a.c[j]; sh_w[q][j] = a.w[j]; }
               }
-            for (int j = 0; j < kSlots; ++j) sh_cand[j] = sh_c[0][j];
+            for (int j = 0; j < kSlots; ++j) { sh_cand[j] = sh_c[0][j]; sh_tot[j] = sh_w[0][j]; }
+            sh_flag2 = d ? 1u : 0u;
           }
           __syncthreads();
 #pragma unroll
-          for (int j = 0; j < kSlots; ++j) cand[j] = sh_cand[j];
+          for (int j = 0; j < kSlots; ++j) { cand[j] = sh_cand[j]; cw[j] = sh_tot[j]; }
+          any_dec = sh_flag2 != 0u;
+          __syncthreads();
         }
         float acc[kSlots];
+        if (!any_dec) {  // merged weights are the exact tree-ordered sums
+#pragma unroll
+          for (int j = 0; j < kSlots; ++j) acc[j] = cw[j];
+        } else {
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
-        for (uint64_t i = b + gl; i < e; i += P) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          const uint32_t c = s.lab[v];
-          const float w = edge_w(g, i);
+        for_edges<UNIT, P>(g, s.lab, u, b + gl, e, [&](uint32_t c, float w) {
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == c ? acc[j] + w : acc[j];
-        }
-        wave_sum8(acc);
+        });
+        group_sum8<W>(acc);
         if (GW > 1) {
           if (lane == 0) {
 #pragma unroll
@@ -252,19 +353,17 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, c
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = sh_tot[j];
         }
+        }
         argmax8(cand, acc, cb, wb);
       } else {
         uint32_t mc = kEmpty;
         float mw = 0.0f;
-        for (uint64_t i = b + gl; i < e; i += P) {
-          const uint32_t v = g.edg[i];
-          if (v == u) continue;
-          bm_step(mc, mw, s.lab[v], edge_w(g, i));
-        }
-        wave_bm_tree(mc, mw);
+        for_edges<UNIT, P>(g, s.lab, u, b + gl, e, [&](uint32_t c, float w) { bm_step(mc, mw, c, w); });
+        group_bm_tree<W>(mc, mw);
         if (GW == 1) {
-          cb = __shfl(mc, 0);
-          wb = __shfl(mw, 0);
+          const int leader = lane & ~(W - 1);
+          cb = __shfl(mc, leader);
+          wb = __shfl(mw, leader);
         } else {
           if (lane == 0) { sh_c[gw][0] = mc; sh_w[gw][0] = mw; }
           __syncthreads();
@@ -280,29 +379,35 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, c
           wb = sh_tot[0];
         }
       }
-      const bool moved = wb > 0.0f && cb != d && !(pickless && cb > d);
+      moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       if (gl == 0) {
         s.aff[u] = 0;
         sa += 1;
         se += deg;
       }
-      if (moved) {
-        if (SYNC) {
-          if (gl == 0) {
-            const unsigned long long pos = atomicAdd(&s.ctr->nmoves, 1ull);
-            s.moves[pos] = ((uint64_t)u << 32) | cb;
-          }
-        } else {
-          if (gl == 0) {
-            s.lab[u] = cb;
-            sc += 1;
-            sx += deg;
-          }
-          for (uint64_t i = b + gl; i < e; i += P) {
-            const uint32_t v = g.edg[i];
-            if (v != u) s.aff[v] = 1;
-          }
+      if (!SYNC && moved) {
+        if (gl == 0) {
+          s.lab[u] = cb;
+          sc += 1;
+          sx += deg;
         }
+        mark_neighbours(g, s.aff, u, b + gl, e, P);
+      }
+    }
+    if (SYNC) {
+      if (GW == 1) {
+        // wave-aggregated append of the moved vertices (one entry per group leader)
+        const bool lead = gl == 0 && moved;
+        const unsigned long long mask = __ballot(lead);
+        if (mask != 0ull) {
+          unsigned long long wbase = 0;
+          if (lane == 0) wbase = atomicAdd(&s.ctr->nmoves, (unsigned long long)__popcll(mask));
+          wbase = __shfl(wbase, 0);
+          if (lead) s.moves[wbase + lanes_below(mask)] = ((uint64_t)u << 32) | cb;
+        }
+      } else if (threadIdx.x == 0 && moved) {
+        const unsigned long long pos = atomicAdd(&s.ctr->nmoves, 1ull);
+        s.moves[pos] = ((uint64_t)u << 32) | cb;
       }
     }
     if (GW > 1) __syncthreads();
@@ -336,10 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
     for (int j = 0; j < cnt; ++j) {
       const uint32_t uj = __shfl(u, j);
       const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
-      for (uint64_t i = bj + lane; i < ej; i += 64) {
-        const uint32_t v = g.edg[i];
-        if (v != uj) s.aff[v] = 1;
-      }
+      mark_neighbours(g, s.aff, uj, bj + lane, ej, 64);
     }
   }
   flush_stats(s.ctr, 0, 0, 0, sx);
@@ -367,9 +469,10 @@ __global__ void k_finish(DevCounters *ctr, uint64_t n, double tolerance, int max
   ctr->nmoves = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_init_labels(uint32_t *lab, const uint32_t *init, uint64_t n) {
-  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (uint64_t)gridDim.x * kBlock)
-    lab[u] = init ? init[u] : (uint32_t)u;
+__global__ __launch_bounds__(kBlock) void k_init_labels(uint32_t *lab, const uint32_t *init, const uint32_t *perm,
+                                                        uint64_t n) {
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < n; p += (uint64_t)gridDim.x * kBlock)
+    lab[p] = init ? init[p] : perm[p];
 }
 
 __global__ __launch_bounds__(kBlock) void k_mark_all(uint8_t *aff, uint64_t n, DevCounters *ctr, int max_iterations) {
@@ -400,40 +503,60 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
-// per vertex: K_u = sum of weights, intra_u = weight to same-label neighbours; tot[c] += K_u
+// Edge-chunked pass: each thread owns kModChunk consecutive CSR positions, finds the source
+// vertex of its first position by binary search over the offsets, and walks forward.
+// intra += w for same-label endpoints; tot[label[u]] += w (one atomic per source run).
+constexpr uint64_t kModChunk = 64;
+
 __global__ __launch_bounds__(kBlock) void k_mod_scan(DevGraph g, const uint32_t *__restrict__ lab, double *ctot,
                                                      uint8_t *seen, double *out) {
-  double si = 0.0, sk = 0.0;
-  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < g.n; u += (uint64_t)gridDim.x * kBlock) {
-    const uint32_t c = lab[u];
-    double ku = 0.0, iu = 0.0;
-    for (uint64_t i = g.off[u]; i < g.off[u + 1]; ++i) {
-      const double w = (double)edge_w(g, i);
-      ku += w;
-      if (lab[g.edg[i]] == c) iu += w;
+  double si = 0.0;
+  const uint64_t nchunks = (g.m + kModChunk - 1) / kModChunk;
+  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < nchunks; t += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i0 = t * kModChunk, i1 = i0 + kModChunk < g.m ? i0 + kModChunk : g.m;
+    // source of position i0: the last u with off[u] <= i0 (and off[u+1] > i0)
+    uint64_t lo = 0, hi = g.n;  // invariant: off[lo] <= i0 < off[hi]
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (g.off[mid] <= i0) lo = mid;
+      else hi = mid;
     }
-    seen[c] = 1;
-    if (ku != 0.0) atomicAdd(&ctot[c], ku);
-    si += iu;
-    sk += ku;
+    uint64_t u = lo, uend = g.off[u + 1];
+    uint32_t c = lab[u];
+    double run = 0.0;
+    for (uint64_t i = i0; i < i1; ++i) {
+      while (i >= uend) {
+        if (run != 0.0) atomicAdd(&ctot[c], run);
+        run = 0.0;
+        ++u;
+        uend = g.off[u + 1];
+        c = lab[u];
+      }
+      const double w = (double)edge_w(g, i);
+      run += w;
+      if (lab[g.edg[i]] == c) si += w;
+    }
+    if (run != 0.0) atomicAdd(&ctot[c], run);
+  }
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < g.n; u += (uint64_t)gridDim.x * kBlock) {
+    seen[lab[u]] = 1;
   }
   si = wave_sum_f64(si);
-  sk = wave_sum_f64(sk);
-  if (lane_id() == 0) {
-    atomicAdd(&out[0], si);
-    atomicAdd(&out[1], sk);
-  }
+  if (lane_id() == 0) atomicAdd(&out[0], si);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mod_sum(const double *ctot, const uint8_t *seen, uint64_t n, double *out) {
-  double sq = 0.0, cnt = 0.0;
+  double sq = 0.0, cnt = 0.0, tw = 0.0;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) {
     sq += ctot[i] * ctot[i];
+    tw += ctot[i];
     cnt += seen[i] ? 1.0 : 0.0;
   }
   sq = wave_sum_f64(sq);
   cnt = wave_sum_f64(cnt);
+  tw = wave_sum_f64(tw);
   if (lane_id() == 0) {
+    atomicAdd(&out[1], tw);
     atomicAdd(&out[2], sq);
     atomicAdd(&out[3], cnt);
   }
@@ -450,43 +573,62 @@ constexpr unsigned kGridCap = 8192;
 
 }  // namespace
 
-void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *init, hipStream_t st) {
-  k_init_labels<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.lab, init, g.n);
+void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *init, const uint32_t *perm,
+                        hipStream_t st) {
+  k_init_labels<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.lab, init, perm, g.n);
 }
 
 void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, hipStream_t st) {
   k_mark_all<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.aff, g.n, s.ctr, max_iterations);
 }
 
-template <bool SYNC, bool MG>
-static void launch_moves(const DevGraph &g, const DevState &s, const ClassLists &cl, const LoopParams &p,
+template <bool SYNC, bool MG, bool UNIT>
+static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
                          hipStream_t st) {
-  const int S = cl.sweeps;
-  for (int w = 0; w < S; ++w) {
-    const uint64_t b0 = cl.seg[w], e0 = cl.seg[w + 1];
-    const uint64_t b1 = cl.seg[S + w], e1 = cl.seg[S + w + 1];
-    const uint64_t b2 = cl.seg[2 * S + w], e2 = cl.seg[2 * S + w + 1];
-    if (e0 > b0)
-      k_move_lane<SYNC, MG><<<grid_for(e0 - b0, kBlock, kGridCap), kBlock, 0, st>>>(g, s, cl.list + b0, e0 - b0,
-                                                                                    p.pickstep);
-    if (e1 > b1)
-      k_move_group<SYNC, MG, 64><<<grid_for(e1 - b1, kBlock / 64, kGridCap), kBlock, 0, st>>>(
-          g, s, cl.list + b1, e1 - b1, p.pickstep);
-    if (e2 > b2)
-      k_move_group<SYNC, MG, 256><<<grid_for(e2 - b2, 1, kGridCap), kBlock, 0, st>>>(g, s, cl.list + b2, e2 - b2,
-                                                                                     p.pickstep);
+  for (const MoveLaunch &L : sc.launches) {
+    const uint64_t items = L.ranges.acc[L.ranges.count];
+    if (items == 0) continue;
+    switch (L.lanes) {
+      case 1:
+        k_move_lane<SYNC, MG, UNIT><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
+                                                                                             p.pickstep);
+        break;
+      case 4:
+        k_move_group<SYNC, MG, UNIT, 4><<<grid_for(items, kBlock / 4, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
+                                                                                                p.pickstep);
+        break;
+      case 16:
+        k_move_group<SYNC, MG, UNIT, 16><<<grid_for(items, kBlock / 16, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
+                                                                                                  p.pickstep);
+        break;
+      case 64:
+        k_move_group<SYNC, MG, UNIT, 64><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
+                                                                                                  p.pickstep);
+        break;
+      default:
+        k_move_group<SYNC, MG, UNIT, 256><<<grid_for(items, 1, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
+                                                                                         p.pickstep);
+        break;
+    }
   }
 }
 
-void launch_iteration(const DevGraph &g, const DevState &s, const ClassLists &cl, const LoopParams &p,
+template <bool SYNC, bool MG>
+static void launch_moves_w(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
+                           hipStream_t st) {
+  if (g.wei) launch_moves<SYNC, MG, false>(g, s, sc, p, st);
+  else launch_moves<SYNC, MG, true>(g, s, sc, p, st);
+}
+
+void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
                       hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1) {
   if (ev_move0) (void)hipEventRecord(ev_move0, st);
   if (p.sync) {
-    if (p.mg) launch_moves<true, true>(g, s, cl, p, st);
-    else launch_moves<true, false>(g, s, cl, p, st);
+    if (p.mg) launch_moves_w<true, true>(g, s, cl, p, st);
+    else launch_moves_w<true, false>(g, s, cl, p, st);
   } else {
-    if (p.mg) launch_moves<false, true>(g, s, cl, p, st);
-    else launch_moves<false, false>(g, s, cl, p, st);
+    if (p.mg) launch_moves_w<false, true>(g, s, cl, p, st);
+    else launch_moves_w<false, false>(g, s, cl, p, st);
   }
   if (ev_move1) (void)hipEventRecord(ev_move1, st);
   if (p.sync) k_apply<<<grid_for(g.n, kBlock, 4096), kBlock, 0, st>>>(g, s);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "rlc.h"
 
@@ -34,13 +35,25 @@ struct DevState {
   DevCounters *ctr;
 };
 
-// Vertex order of an iteration: list[seg[c * sweeps + s] .. seg[c * sweeps + s + 1]) holds the
-// vertices of degree class c (0 lane-per-vertex, 1 wave-per-vertex, 2 workgroup-per-vertex) in
-// sub-sweep s. Sub-sweeps run one after the other; inside one, the three classes run in turn.
-struct ClassLists {
-  const uint32_t *list;
-  const uint64_t *seg;  // host array of 3 * sweeps + 1 entries
-  int sweeps;
+// A set of position ranges [first[r], last[r]) of the degree-descending layout; acc[] is the
+// prefix count of the items (vertices, or 64-position chunks for the lane kernel) of the ranges.
+constexpr int kMaxRanges = 64;
+struct RangeSet {
+  uint32_t count;
+  uint64_t first[kMaxRanges];
+  uint64_t last[kMaxRanges];
+  uint64_t acc[kMaxRanges + 1];
+};
+
+// One move launch: `lanes` lanes per vertex (1, 4, 16, 64 or 256) over a range set.
+struct MoveLaunch {
+  int lanes;
+  RangeSet ranges;
+};
+
+// The move launches of one iteration, in order.
+struct Schedule {
+  std::vector<MoveLaunch> launches;
 };
 
 struct LoopParams {
@@ -49,15 +62,16 @@ struct LoopParams {
   int pickstep;
   double tolerance;
   int max_iterations;
-  int block_parts;  // parts of the workgroup class: 256
 };
 
-// labels = init ? init : ids; flags = 1; counters reset (done = max_iterations <= 0)
-void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *init, hipStream_t st);
+// flags = 1; counters reset (done = max_iterations <= 0)
+// lab[p] = init ? init[p] : perm[p] (label values are original vertex ids)
+void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *init, const uint32_t *perm,
+                        hipStream_t st);
 void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, hipStream_t st);
 // One iteration: move launches for each non-empty class, the apply pass (synchronous mode) and
 // the finish step. ev_move0/ev_move1 (may be null) bracket the move launches.
-void launch_iteration(const DevGraph &g, const DevState &s, const ClassLists &cl, const LoopParams &p,
+void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
                       hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1);
 // FP64 modularity terms: out[0] = sum of intra-community weight, out[1] = total weight,
 // out[2] = sum over labels of tot^2, out[3] = number of distinct labels. Needs labels < n.

@@ -16,13 +16,21 @@ constexpr int kSlots = 8;
 // (inc/rakLowmem.hxx:63-89): add w to the slot holding c; otherwise take the highest-index
 // zero-weight slot; otherwise subtract w from every slot, clamped at zero. (The reference GPU
 // variant, inc/sketchCuda.hxx:73-108, takes the lowest free slot and does not clamp.)
+//
+// `dec` records whether any decrement happened. Without one, no label was ever evicted and each
+// slot's weight is the sum of its label's weights in scan order, starting from the insert,
+// which is bit for bit what the reference's rescan (inc/rakLowmem.hxx:154-162) recomputes; the
+// rescan pass is then skipped. The same holds across a merge tree whose absorbs never decrement:
+// absorb adds part sums in the order the tree-ordered rescan adds them.
 struct MG8 {
   uint32_t c[kSlots];
   float w[kSlots];
+  bool dec;
 
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) { c[i] = kEmpty; w[i] = 0.0f; }
+    dec = false;
   }
 
   __device__ __forceinline__ void accumulate(uint32_t lab, float wt) {
@@ -34,19 +42,21 @@ struct MG8 {
       has = has || m;
     }
     if (has) return;
+    // highest-index zero-weight slot, or -1; then either insert there or decrement all slots.
+    // Branch-free: every slot computes both outcomes and selects. The clamp max(x, 0) is
+    // fmaxf: weights are finite and never -0 here (x - y with x >= 0, y > 0 rounds to +0 when
+    // equal), so it equals the reference's std::max(x, 0).
     int f = -1;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) f = (w[i] == 0.0f) ? i : f;
-    if (f >= 0) {
+    const bool full = f < 0;
+    dec = dec || full;
 #pragma unroll
-      for (int i = 0; i < kSlots; ++i)
-        if (i == f) { c[i] = lab; w[i] = wt; }
-    } else {
-#pragma unroll
-      for (int i = 0; i < kSlots; ++i) {
-        const float x = w[i] - wt;
-        w[i] = x < 0.0f ? 0.0f : x;
-      }
+    for (int i = 0; i < kSlots; ++i) {
+      const bool sel = i == f;
+      const float x = fmaxf(w[i] - wt, 0.0f);
+      c[i] = sel ? lab : c[i];
+      w[i] = sel ? wt : (full ? x : w[i]);
     }
   }
 
@@ -91,12 +101,14 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Merge tree over the 64 lanes of a wave: at level k (1, 2, 4, .., 32) lane p with p % 2k == 0
-// absorbs lane p+k. The result is in lane 0; other lanes hold partial sketches.
-__device__ __forceinline__ void wave_merge_tree(MG8 &s) {
+// Merge tree over groups of W lanes (W a power of two <= 64, groups aligned in the wave): at
+// level k (1, 2, 4, .., W/2) lane p of a group with p % 2k == 0 absorbs lane p+k. The result is
+// in the group's first lane; other lanes hold partial sketches.
+template <int W>
+__device__ __forceinline__ void group_merge_tree(MG8 &s) {
   const int lane = lane_id();
 #pragma unroll
-  for (int k = 1; k < 64; k <<= 1) {
+  for (int k = 1; k < W; k <<= 1) {
     uint32_t oc[kSlots];
     float ow[kSlots];
 #pragma unroll
@@ -108,23 +120,25 @@ __device__ __forceinline__ void wave_merge_tree(MG8 &s) {
   }
 }
 
-// Boyer-Moore max tree over the wave: lane p takes lane p+k's candidate if strictly heavier.
-__device__ __forceinline__ void wave_bm_tree(uint32_t &mc, float &mw) {
+// Boyer-Moore max tree over groups of W lanes: lane p takes lane p+k's candidate if strictly heavier.
+template <int W>
+__device__ __forceinline__ void group_bm_tree(uint32_t &mc, float &mw) {
   const int lane = lane_id();
 #pragma unroll
-  for (int k = 1; k < 64; k <<= 1) {
+  for (int k = 1; k < W; k <<= 1) {
     const uint32_t oc = __shfl_down(mc, k);
     const float ow = __shfl_down(mw, k);
     if ((lane & (2 * k - 1)) == 0 && ow > mw) { mc = oc; mw = ow; }
   }
 }
 
-// Butterfly sum over the wave with masks 1, 2, .., 32: every lane ends with the value of the
-// binary tree (p absorbs p+k) evaluated at its group's root; float addition is commutative, so
-// all lanes agree bit for bit.
-__device__ __forceinline__ void wave_sum8(float (&a)[kSlots]) {
+// Butterfly sum over groups of W lanes with masks 1, 2, .., W/2: every lane of a group ends with
+// the value of the binary tree (p absorbs p+k) at the group's root; float addition is
+// commutative, so all lanes of the group agree bit for bit.
+template <int W>
+__device__ __forceinline__ void group_sum8(float (&a)[kSlots]) {
 #pragma unroll
-  for (int k = 1; k < 64; k <<= 1)
+  for (int k = 1; k < W; k <<= 1)
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) a[j] = a[j] + __shfl_xor(a[j], k);
 }

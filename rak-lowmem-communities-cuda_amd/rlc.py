@@ -36,9 +36,8 @@ class Options(C.Structure):
     _fields_ = [
         ("repeat", C.c_int32), ("tolerance", C.c_double), ("max_iterations", C.c_int32),
         ("slots", C.c_int32), ("mode", C.c_int32), ("pickstep", C.c_int32),
-        ("merge_degree", C.c_int32), ("merge_parts", C.c_int32), ("wave_degree", C.c_int32),
-        ("block_degree", C.c_int32), ("timing", C.c_int32), ("sweeps", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("merge_degree", C.c_int32), ("merge_parts", C.c_int32), ("group_degree", C.c_int32 * 4),
+        ("timing", C.c_int32), ("sweeps", C.c_int32), ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -55,7 +54,8 @@ class RunStats(C.Structure):
 
 class GraphInfo(C.Structure):
     _fields_ = [("n", C.c_uint64), ("m", C.c_uint64), ("max_degree", C.c_uint64),
-                ("device_bytes", C.c_uint64), ("unit_weights", C.c_int32), ("offsets_bytes", C.c_int32)]
+                ("device_bytes", C.c_uint64), ("unit_weights", C.c_int32), ("offsets_bytes", C.c_int32),
+                ("segments", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Csr(C.Structure):
@@ -79,6 +79,7 @@ SIGNATURES = [
     ("rlc_graph_destroy", None, [_P]),
     ("rlc_run", C.c_int, [_P, _P, _P, C.POINTER(Options), _P, C.POINTER(RunStats)]),
     ("rlc_graph_labels_device", _P, [_P]),
+    ("rlc_graph_vertex_order", C.c_int, [_P, _P]),
     ("rlc_modularity", C.c_int, [_P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("rlc_gen_sbm", C.c_int, [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Csr)]),
     ("rlc_gen_rmat", C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_float,
@@ -175,6 +176,8 @@ def options(**kw) -> Options:
     for k, v in kw.items():
         if not hasattr(o, k):
             raise KeyError(k)
+        if k == "group_degree":
+            v = (C.c_int32 * 4)(*v)
         setattr(o, k, v)
     return o
 
@@ -253,6 +256,11 @@ class DeviceGraph:
         ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint32)
         _check(lib().rlc_run(self.ctx.h, self.h, _ptr(ini), C.byref(o), _ptr(out), C.byref(st)))
         return RakResult(out, st.iterations, st.time_ms, st.marking_ms, st.init_ms, st.memory_gb, st)
+
+    def vertex_order(self) -> np.ndarray:
+        o = np.empty(self.n, np.uint32)
+        _check(lib().rlc_graph_vertex_order(self.h, _ptr(o)))
+        return o
 
     def modularity(self, labels: np.ndarray | None = None):
         q, k = C.c_double(), C.c_uint64()

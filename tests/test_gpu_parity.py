@@ -52,7 +52,7 @@ def test_cfg1_sync_bit_exact(ctx, slots):
 
 
 @pytest.mark.parametrize("slots", [8, 1])
-@pytest.mark.parametrize("merge", [(0, 64), (8, 64), (40, 256), (1, 64), (1, 256)])
+@pytest.mark.parametrize("merge", [(0, 64), (8, 64), (40, 256), (1, 64), (1, 256), (3, 4), (20, 16)])
 def test_sync_merge_tree_bit_exact(ctx, slots, merge):
     g = with_hubs(3000, hubs=6, hub_degree=1500, base_degree=12, seed=7)
     assert_sync_parity(ctx, g, slots=slots, merge_degree=merge[0], merge_parts=merge[1])
@@ -157,25 +157,29 @@ def async_quality(ctx, g, slots=8, **kw):
     return r, q_dev
 
 
-@pytest.mark.parametrize("wave,block", [(0, 0), (4, 64), (1, 1 << 30), (1 << 30, 1 << 30), (2, 2)])
-def test_async_cfg1_quality(ctx, wave, block):
+GROUPS = [(0, 0, 0, 0), (4, 8, 16, 64), (1, 1, 1, 1 << 30), (1 << 30,) * 4, (1, 1, 1, 1), (0, 0, 2, 2),
+          (1, 1 << 30, 1 << 30, 1 << 30)]
+
+
+@pytest.mark.parametrize("groups", GROUPS, ids=[str(i) for i in range(len(GROUPS))])
+def test_async_cfg1_quality(ctx, groups):
     g = cfg1()
-    r, q = async_quality(ctx, g, wave_degree=wave, block_degree=block)
+    r, q = async_quality(ctx, g, group_degree=groups)
     q_ref = O.modularity(g, O.lowmem_omp(g, True)[0])
-    print(f"cfg1 async wave={wave} block={block}: Q={q:.6f} ref={q_ref:.6f} iters={r.iterations}")
-    # the default schedule is held to the 1% gate; forcing low-degree vertices into the wave or
-    # workgroup kernels changes tie-breaking (merge tree instead of one sequential sketch) and
-    # is held to 3%
-    assert q >= q_ref * (0.99 if (wave, block) == (0, 0) else 0.97)
+    print(f"cfg1 async groups={groups}: Q={q:.6f} ref={q_ref:.6f} iters={r.iterations}")
+    # the default schedule is held to the 1% gate; forcing low-degree vertices into the group
+    # kernels changes tie-breaking (merge tree instead of one sequential sketch) and is held to
+    # 3%; the workgroup class is not split into sub-sweeps (hubs go first, all at once), so
+    # forcing every vertex into it behaves like the synchronous mode (Q 0.74 on this graph)
+    all_hub = groups[3] <= 2
+    assert q >= q_ref * (0.99 if groups == (0, 0, 0, 0) else (0.9 if all_hub else 0.97))
 
 
-@pytest.mark.parametrize("sweeps", [1, 4, 16])
-def test_async_sweeps_run(ctx, sweeps):
-    # any sub-sweep count is a valid asynchronous schedule; fewer sweeps behave more like the
-    # synchronous mode (lower quality), so only the default count is held to the 1% gate
+def test_async_single_sweep_runs(ctx):
+    # one sweep is a valid asynchronous schedule, closer to the synchronous mode (lower quality)
     g = cfg1()
-    r, q = async_quality(ctx, g, sweeps=sweeps)
-    print(f"cfg1 async sweeps={sweeps}: Q={q:.6f} iters={r.iterations}")
+    r, q = async_quality(ctx, g, sweeps=1)
+    print(f"cfg1 async sweeps=1: Q={q:.6f} iters={r.iterations}")
     assert q > 0.7
 
 
