@@ -76,10 +76,11 @@ typedef struct rlc_options {
   int32_t merge_degree;   /* SYNC: vertices of degree >= merge_degree use the P-part merge-tree sketch,
                              0 = never (every vertex scans its edges sequentially) [0] */
   int32_t merge_parts;    /* SYNC: P, one of 4, 16, 64, 256 [64] */
-  int32_t group_degree[4];/* ASYNC: vertices of degree >= group_degree[k] are scanned by 4, 16, 64 (one
-                             wavefront) or 256 (one workgroup) lanes for k = 0..3; below
-                             group_degree[0] one lane scans a vertex. Non-decreasing; all zero = the
-                             library defaults {32, 128, 512, 4096} */
+  int32_t group_degree[4];/* ASYNC: vertices of degree >= group_degree[k], k = 0..2, are scanned by 4,
+                             16 or 64 (one wavefront) lanes; below group_degree[0] one lane scans a
+                             vertex; vertices of degree >= group_degree[3] are cut into 4096-edge
+                             chunks, one wavefront per chunk (MG8; 256 lanes per vertex for BM).
+                             Non-decreasing; all zero = the library defaults */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
   int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: vertices are
                              split by (id / 64) % S into S segments, run one after the other so that

@@ -98,6 +98,10 @@ struct rlc_graph {
   hipGraphExec_t hexec = nullptr;
   ExecKey key;
   std::atomic<int> busy{0};
+  // chunk plan of the asynchronous mode (vertices of degree >= plan_degree)
+  void *d_plan = nullptr;
+  uint64_t plan_degree = 0;
+  ChunkPlan plan{};
 
   DevGraph dev() const { return DevGraph{d_off, d_edg, unit ? nullptr : d_wei, n, m}; }
   DevState state() const { return DevState{d_lab, d_aff, d_moves, d_ctr}; }
@@ -122,7 +126,7 @@ void free_graph(rlc_graph *g) {
   if (g->hexec) (void)hipGraphExecDestroy(g->hexec);
   if (g->hgraph) (void)hipGraphDestroy(g->hgraph);
   void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_aff, g->d_moves,
-                  g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout};
+                  g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout, g->d_plan};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &e : g->ev)
@@ -151,6 +155,64 @@ void add_range(RangeSet &rs, uint64_t first, uint64_t last, bool chunks) {
   rs.acc[r + 1] = rs.acc[r] + (chunks ? (last - first + 63) / 64 : last - first);
 }
 
+constexpr uint32_t kChunkEdges = 4096;
+
+// Device tables of the chunked class: every vertex of degree >= t, in layout order, cut into
+// kChunkEdges-edge chunks.
+rlc_status ensure_chunk_plan(rlc_graph *g, uint64_t t) {
+  if (g->d_plan && g->plan_degree == t) return RLC_OK;
+  if (g->d_plan) { (void)hipFree(g->d_plan); g->d_plan = nullptr; }
+  std::vector<uint32_t> cslot, cidx, vpos, vfirst, vcount;
+  const size_t S = g->h_seg.size() - 1;
+  for (size_t q = 0; q < S; ++q) {
+    const uint64_t sb = g->h_seg[q], se = g->h_seg[q + 1];
+    const uint64_t cut = first_below(g, sb, se, t);
+    for (uint64_t p = sb; p < cut; ++p) {
+      const uint64_t deg = g->h_off[p + 1] - g->h_off[p];
+      const uint32_t nc = (uint32_t)((deg + kChunkEdges - 1) / kChunkEdges);
+      const uint32_t slot = (uint32_t)vpos.size();
+      vpos.push_back((uint32_t)p);
+      vfirst.push_back((uint32_t)cslot.size());
+      vcount.push_back(nc);
+      for (uint32_t k = 0; k < nc; ++k) { cslot.push_back(slot); cidx.push_back(k); }
+    }
+  }
+  const uint64_t C = cslot.size(), V = vpos.size();
+  // layout of one allocation: 5 uint32 tables, per-slot state, per-chunk partials
+  const uint64_t words = 2 * C + 3 * V + 4 * V + 8 * V * 2 + 8 * C * 2 + C;
+  uint32_t *base = nullptr;
+  rlc_status st = dev_alloc(&base, std::max<uint64_t>(words, 1));
+  if (st != RLC_OK) return st;
+  g->d_plan = base;
+  uint32_t *w = base;
+  auto take = [&](uint64_t k) { uint32_t *r = w; w += k; return r; };
+  ChunkPlan &cp = g->plan;
+  cp.nchunks = C;
+  cp.nslots = (uint32_t)V;
+  cp.chunk_edges = kChunkEdges;
+  uint32_t *d_cslot = take(C), *d_cidx = take(C), *d_vpos = take(V), *d_vfirst = take(V), *d_vcount = take(V);
+  cp.cslot = d_cslot; cp.cidx = d_cidx; cp.vpos = d_vpos; cp.vfirst = d_vfirst; cp.vcount = d_vcount;
+  cp.cnt = take(V); cp.act = take(V); cp.moved = take(V); cp.vdec = take(V);
+  cp.cand = take(8 * V);
+  cp.cw = reinterpret_cast<float *>(take(8 * V));
+  cp.pc = take(8 * C);
+  cp.pw = reinterpret_cast<float *>(take(8 * C));
+  cp.pdec = take(C);
+  auto up = [&](uint32_t *d, const std::vector<uint32_t> &h) {
+    return h.empty() ? hipSuccess : hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, g->stream);
+  };
+  HIP_TRY(up(d_cslot, cslot));
+  HIP_TRY(up(d_cidx, cidx));
+  HIP_TRY(up(d_vpos, vpos));
+  HIP_TRY(up(d_vfirst, vfirst));
+  HIP_TRY(up(d_vcount, vcount));
+  if (V) HIP_TRY(hipMemsetAsync(cp.cnt, 0, V * 4, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->plan_degree = t;
+  g->key.valid = false;  // captured launches embed the plan
+  return RLC_OK;
+}
+
 RangeSet empty_ranges() {
   RangeSet rs;
   std::memset(&rs, 0, sizeof(rs));
@@ -163,7 +225,8 @@ RangeSet empty_ranges() {
 //         one lane launch over the rest.
 //   ASYNC: hubs (256 lanes) of every segment first, then segment after segment its 64-, 16-, 4-lane
 //         classes and its lane class (or all segments in one launch per class when sweeps == 1).
-Schedule build_schedule(const rlc_graph *g, const rlc_options *o) {
+Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
+  *st = RLC_OK;
   Schedule sc;
   const size_t S = g->h_seg.size() - 1;
   if (o->mode == RLC_MODE_SYNC) {
@@ -192,9 +255,16 @@ Schedule build_schedule(const rlc_graph *g, const rlc_options *o) {
     for (int k = 0; k < 4; ++k) cuts[s][k + 1] = first_below(g, sb, se, t[3 - k]);
     cuts[s][5] = se;
   }
-  MoveLaunch hubs{256, empty_ranges()};
-  for (size_t s = 0; s < S; ++s) add_range(hubs.ranges, cuts[s][0], cuts[s][1], false);
-  sc.launches.push_back(hubs);
+  if (o->slots == RLC_SKETCH_MG8) {
+    // the top class is cut into chunks (one wavefront each)
+    if ((*st = ensure_chunk_plan(g, t[3])) != RLC_OK) return sc;
+    sc.chunked = true;
+    sc.plan = g->plan;
+  } else {
+    MoveLaunch hubs{256, empty_ranges()};
+    for (size_t s = 0; s < S; ++s) add_range(hubs.ranges, cuts[s][0], cuts[s][1], false);
+    sc.launches.push_back(hubs);
+  }
   const bool one_sweep = o->sweeps == 1;
   for (size_t s = 0; s < (one_sweep ? 1 : S); ++s) {
     for (int c = 1; c < 5; ++c) {
@@ -421,7 +491,8 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   }
   const bool sync = o->mode == RLC_MODE_SYNC;
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
-  const Schedule cl = build_schedule(g, o);
+  const Schedule cl = build_schedule(g, o, &st);
+  if (st != RLC_OK) return st;
 
   const DevGraph dg = g->dev();
   const DevState ds = g->state();

@@ -415,6 +415,175 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
   flush_stats(s.ctr, sa, se, sc, sx);
 }
 
+// ---- asynchronous mode: vertices cut into chunks, one wavefront per chunk -------------------
+// Five launches, with kernel boundaries as the only synchronisation (on this GPU a device-scope
+// fence writes back or invalidates the whole L2 of an XCD, far more costly than a launch):
+//   scan    per chunk: 64 strided lane sketches merged by the wave tree -> partial sketch
+//   merge   per vertex: partial sketches merged (wave tree, 64 at a time) -> candidates
+//   rescan  per chunk: exact weights of the candidates (only when a decrement happened)
+//   choose  per vertex: totals, argmax, label write, flags, counters
+//   mark    per chunk: flags of the neighbours of moved vertices
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_chunk_scan(DevGraph g, DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t c = wave_global; c < cp.nchunks; c += nwaves) {
+    const uint32_t slot = cp.cslot[c], ci = cp.cidx[c];
+    const uint32_t u = cp.vpos[slot];
+    const uint32_t act = uniform_u32(s.aff[u]);
+    if (ci == 0 && lane == 0) cp.act[slot] = act;
+    if (!act) continue;
+    const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
+    const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
+    const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
+    MG8 sk;
+    sk.clear();
+    for_edges<UNIT, 64>(g, s.lab, u, b + lane, e, [&](uint32_t cl, float w) { sk.accumulate(cl, w); });
+    group_merge_tree<64>(sk);
+    const bool dec = __ballot(sk.dec) != 0ull;
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) { cp.pc[c * kSlots + j] = sk.c[j]; cp.pw[c * kSlots + j] = sk.w[j]; }
+      cp.pdec[c] = dec ? 1u : 0u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_chunk_merge(DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t slot = wave_global; slot < cp.nslots; slot += nwaves) {
+    if (!cp.act[slot]) continue;
+    const uint32_t f0 = cp.vfirst[slot], nc = cp.vcount[slot];
+    MG8 acc;
+    acc.clear();
+    bool anyd = false;
+    for (uint32_t r = 0; r < nc; r += 64) {
+      MG8 part;
+      part.clear();
+      const uint32_t k = r + (uint32_t)lane;
+      if (k < nc) {
+        const uint64_t pcid = (uint64_t)f0 + k;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) { part.c[j] = cp.pc[pcid * kSlots + j]; part.w[j] = cp.pw[pcid * kSlots + j]; }
+        anyd = anyd || cp.pdec[pcid] != 0u;
+      }
+      group_merge_tree<64>(part);
+      if (lane == 0) acc.absorb(part.c, part.w);
+      anyd = anyd || part.dec;
+    }
+    anyd = __ballot(anyd || acc.dec) != 0ull;
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) { cp.cand[slot * kSlots + j] = acc.c[j]; cp.cw[slot * kSlots + j] = acc.w[j]; }
+      cp.vdec[slot] = anyd ? 1u : 0u;
+    }
+  }
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_chunk_rescan(DevGraph g, DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t c = wave_global; c < cp.nchunks; c += nwaves) {
+    const uint32_t slot = cp.cslot[c], ci = cp.cidx[c];
+    if (!cp.act[slot] || !cp.vdec[slot]) continue;
+    const uint32_t u = cp.vpos[slot];
+    uint32_t cand[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) cand[j] = cp.cand[slot * kSlots + j];
+    const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
+    const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
+    const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
+    float acc[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
+    for_edges<UNIT, 64>(g, s.lab, u, b + lane, e, [&](uint32_t cl, float w) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == cl ? acc[j] + w : acc[j];
+    });
+    group_sum8<64>(acc);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) cp.pw[c * kSlots + j] = acc[j];
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_chunk_choose(DevGraph g, DevState s, ChunkPlan cp, int pickstep) {
+  if (s.ctr->done) return;
+  const uint32_t iter = s.ctr->iter;
+  const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
+  for (uint64_t slot = wave_global; slot < cp.nslots; slot += nwaves) {
+    if (!cp.act[slot]) continue;
+    const uint32_t u = cp.vpos[slot];
+    uint32_t cand[kSlots];
+    float tot[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) cand[j] = cp.cand[slot * kSlots + j];
+    if (cp.vdec[slot]) {
+      const uint32_t f0 = cp.vfirst[slot], nc = cp.vcount[slot];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) tot[j] = 0.0f;
+      for (uint32_t r = 0; r < nc; r += 64) {
+        float part[kSlots];
+        const uint32_t k = r + (uint32_t)lane;
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) part[j] = k < nc ? cp.pw[((uint64_t)f0 + k) * kSlots + j] : 0.0f;
+        group_sum8<64>(part);
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) tot[j] += part[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) tot[j] = cp.cw[slot * kSlots + j];
+    }
+    uint32_t cb;
+    float wb;
+    argmax8(cand, tot, cb, wb);
+    if (lane == 0) {
+      const uint32_t d = s.lab[u];
+      const bool moved = wb > 0.0f && cb != d && !(pickless && cb > d);
+      const uint64_t deg = g.off[(uint64_t)u + 1] - g.off[u];
+      s.aff[u] = 0;
+      if (moved) s.lab[u] = cb;
+      cp.moved[slot] = moved ? 1u : 0u;
+      sa += 1;
+      se += deg;
+      sc += moved ? 1 : 0;
+      sx += moved ? deg : 0;
+    }
+  }
+  flush_stats(s.ctr, sa, se, sc, sx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t c = wave_global; c < cp.nchunks; c += nwaves) {
+    const uint32_t slot = cp.cslot[c], ci = cp.cidx[c];
+    if (!cp.act[slot] || !cp.moved[slot]) continue;
+    const uint32_t u = cp.vpos[slot];
+    const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
+    const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
+    const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
+    mark_neighbours(g, s.aff, u, b + lane, e, 64);
+  }
+}
+
 // ---- synchronous mode: commit the change list ------------------------------------------------
 // Lane per change for the label write; the wave then marks each change's neighbours together.
 __global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
@@ -585,6 +754,15 @@ void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, h
 template <bool SYNC, bool MG, bool UNIT>
 static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
                          hipStream_t st) {
+  if (!SYNC && MG && sc.chunked && sc.plan.nchunks > 0) {
+    const unsigned gr = grid_for(sc.plan.nchunks, kBlock / 64, kGridCap);
+    const unsigned gv = grid_for(sc.plan.nslots, kBlock / 64, kGridCap);
+    k_chunk_scan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_chunk_merge<<<gv, kBlock, 0, st>>>(s, sc.plan);
+    k_chunk_rescan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_chunk_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
+    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+  }
   for (const MoveLaunch &L : sc.launches) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
     if (items == 0) continue;

@@ -51,9 +51,34 @@ struct MoveLaunch {
   RangeSet ranges;
 };
 
-// The move launches of one iteration, in order.
+// Asynchronous mode: vertices of very high degree are cut into chunks of `chunk_edges`
+// consecutive edges, one wavefront per chunk. Chunks of a vertex are consecutive in the chunk
+// table; per-vertex values live at the vertex's slot, per-chunk values at the chunk id.
+struct ChunkPlan {
+  uint64_t nchunks;
+  uint32_t nslots;
+  uint32_t chunk_edges;
+  const uint32_t *cslot;   // chunk -> vertex slot
+  const uint32_t *cidx;    // chunk -> index of the chunk inside its vertex
+  const uint32_t *vpos;    // slot -> vertex position
+  const uint32_t *vfirst;  // slot -> first chunk
+  const uint32_t *vcount;  // slot -> number of chunks
+  uint32_t *cnt;           // slot -> chunks finished in the current pass (reset by the last)
+  uint32_t *act;           // slot -> vertex processed in this iteration
+  uint32_t *moved;         // slot -> vertex changed label in this iteration
+  uint32_t *vdec;          // slot -> a decrement happened somewhere (rescan needed)
+  uint32_t *cand;          // slot * 8 -> merged sketch labels
+  float *cw;               // slot * 8 -> merged sketch weights
+  uint32_t *pc;            // chunk * 8 -> partial sketch labels
+  float *pw;               // chunk * 8 -> partial sketch weights / rescan partial sums
+  uint32_t *pdec;          // chunk -> partial sketch decremented
+};
+
+// The move launches of one iteration, in order (chunked vertices first when present).
 struct Schedule {
   std::vector<MoveLaunch> launches;
+  bool chunked = false;
+  ChunkPlan plan{};
 };
 
 struct LoopParams {
