@@ -1,0 +1,168 @@
+// rak_lowmem_hip.hxx — C++ drop-in for the reference's memory-efficient label propagation API,
+// implemented over the C ABI of rlc.h (librlc.so, HIP kernels for gfx950).
+//
+// Mirrors (puzzlef/rak-lowmem-communities-cuda):
+//   RakOptions                          inc/rak.hxx:19-40
+//   RakResult<K>                        inc/rak.hxx:53-97
+//   rakLowmemStaticCuda<SLOTS,...>(x,o) inc/rakLowmemCuda.hxx:624-632
+//   rakLowmemCuda(x, init, o)           rakLowmemInvokeCuda with an initial membership
+//                                       (inc/rakLowmemCuda.hxx:538-607, its `fi` hook at :590)
+//   G                                   the reference's graph concept (inc/Graph.hxx:59-168): key_type,
+//                                       edge_value_type, span(), order(), hasVertex(u),
+//                                       forEachVertexKey(f), forEachEdge(u, f)
+//
+// Differences from the reference, by design:
+//   - errors throw rlc_hip::Error (the reference prints and exits, inc/_cuda.hxx:221-235);
+//   - membership values are vertex ids of the caller's graph (the reference returns ids of its
+//     internally permuted vertex order, inc/rakLowmemCuda.hxx:563-568);
+//   - the RESCAN / TRYWARP / TRYMERGE template knobs are accepted and ignored: the library always
+//     rescans when a sketch decremented, and picks lane / group / chunk kernels by degree;
+//   - a run may be deterministic (RLC_MODE_SYNC) through the rlc_options overloads.
+#pragma once
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "rlc.h"
+
+/** Options for RAK algorithm (inc/rak.hxx:19-40). */
+struct RakOptions {
+  int repeat;
+  double tolerance;
+  int maxIterations;
+  RakOptions(int repeat = 1, double tolerance = 0.05, int maxIterations = 20)
+      : repeat(repeat), tolerance(tolerance), maxIterations(maxIterations) {}
+};
+
+/** Result of RAK algorithm (inc/rak.hxx:53-97). */
+template <class K>
+struct RakResult {
+  std::vector<K> membership;
+  int iterations;
+  float time;
+  float markingTime;
+  float initializationTime;
+  float memory;
+  RakResult(std::vector<K> &&membership, int iterations = 0, float time = 0, float markingTime = 0,
+            float initializationTime = 0, float memory = 0)
+      : membership(std::move(membership)), iterations(iterations), time(time), markingTime(markingTime),
+        initializationTime(initializationTime), memory(memory) {}
+};
+
+namespace rlc_hip {
+
+struct Error : std::runtime_error {
+  rlc_status status;
+  Error(rlc_status s, const std::string &what) : std::runtime_error(what), status(s) {}
+};
+
+inline void check(rlc_status s, const char *where) {
+  if (s != RLC_OK) throw Error(s, std::string(where) + ": " + rlc_status_string(s) + ": " + rlc_last_error());
+}
+
+/** One context per process on device 0 (the reference uses the current device). */
+inline rlc_context *context() {
+  static std::once_flag once;
+  static std::shared_ptr<rlc_context> ctx;
+  std::call_once(once, [] {
+    rlc_context *c = nullptr;
+    check(rlc_context_create(0, &c), "rlc_context_create");
+    ctx.reset(c, rlc_context_destroy);
+  });
+  return ctx.get();
+}
+
+/** Compact CSR of a graph G: existing vertices in id order, each adjacency in G's order. */
+template <class G>
+struct Csr {
+  using K = typename G::key_type;
+  std::vector<K> ids;         // compact -> graph id
+  std::vector<uint64_t> pos;  // graph id -> compact (span entries)
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> edg;
+  std::vector<float> wei;
+  bool unit = true;
+
+  explicit Csr(const G &x) {
+    const size_t S = x.span();
+    pos.assign(S, ~uint64_t(0));
+    x.forEachVertexKey([&](auto u) {
+      pos[u] = ids.size();
+      ids.push_back((K)u);
+    });
+    off.assign(ids.size() + 1, 0);
+    for (size_t i = 0; i < ids.size(); ++i) {
+      x.forEachEdge(ids[i], [&](auto v, auto w) {
+        edg.push_back((uint32_t)pos[v]);
+        wei.push_back((float)w);
+        unit = unit && (float)w == 1.0f;
+      });
+      off[i + 1] = edg.size();
+    }
+  }
+};
+
+/** Run on a graph G; init (span entries, indexed by graph id) may be null. */
+template <class G>
+inline RakResult<typename G::key_type> run(const G &x, const std::vector<typename G::key_type> *init,
+                                           const rlc_options &o) {
+  using K = typename G::key_type;
+  Csr<G> c(x);
+  const uint64_t n = c.ids.size();
+  rlc_graph *g = nullptr;
+  check(rlc_graph_upload(context(), n, c.edg.size(), c.off.data(), c.edg.data(), c.unit ? nullptr : c.wei.data(), &g),
+        "rlc_graph_upload");
+  std::unique_ptr<rlc_graph, void (*)(rlc_graph *)> guard(g, rlc_graph_destroy);
+  std::vector<uint32_t> in, out(n);
+  if (init) {
+    in.resize(n);
+    for (uint64_t i = 0; i < n; ++i) in[i] = (uint32_t)(*init)[c.ids[i]];
+  }
+  rlc_run_stats st;
+  check(rlc_run(context(), g, init ? in.data() : nullptr, &o, out.data(), &st), "rlc_run");
+  // membership indexed by graph id (span entries, absent vertices 0 as in the reference);
+  // with the default initial labels (compact ids) values are mapped back to graph ids
+  std::vector<K> membership(x.span());
+  for (uint64_t i = 0; i < n; ++i) membership[c.ids[i]] = init ? (K)out[i] : c.ids[out[i]];
+  return RakResult<K>(std::move(membership), st.iterations, st.time_ms, st.marking_ms, st.init_ms, st.memory_gb);
+}
+
+inline rlc_options options(const RakOptions &o, int slots) {
+  rlc_options r;
+  rlc_options_default(&r);
+  r.repeat = o.repeat;
+  r.tolerance = o.tolerance;
+  r.max_iterations = o.maxIterations;
+  r.slots = slots;
+  return r;
+}
+
+}  // namespace rlc_hip
+
+/** rakLowmemStaticCuda<SLOTS>(x, o): SLOTS=8 Misra-Gries (νMG8-LPA), SLOTS=1 Boyer-Moore (νBM-LPA). */
+template <int SLOTS = 8, bool RESCAN = false, bool TRYWARP = true, bool TRYMERGE = true, class G>
+inline auto rakLowmemStaticCuda(const G &x, const RakOptions &o = {}) {
+  static_assert(SLOTS == 1 || SLOTS == 8, "SLOTS must be 1 (Boyer-Moore) or 8 (Misra-Gries)");
+  return rlc_hip::run(x, nullptr, rlc_hip::options(o, SLOTS));
+}
+
+/** Same, with full control of the library options (mode, sweeps, thresholds, timing). */
+template <class G>
+inline auto rakLowmemStaticCuda(const G &x, const rlc_options &o) {
+  return rlc_hip::run(x, nullptr, o);
+}
+
+/** rakLowmemCuda(x, init, o): start from an initial membership (span entries, graph ids). */
+template <int SLOTS = 8, class G, class K>
+inline auto rakLowmemCuda(const G &x, const std::vector<K> &init, const RakOptions &o = {}) {
+  static_assert(SLOTS == 1 || SLOTS == 8, "SLOTS must be 1 (Boyer-Moore) or 8 (Misra-Gries)");
+  return rlc_hip::run(x, &init, rlc_hip::options(o, SLOTS));
+}
+
+template <class G, class K>
+inline auto rakLowmemCuda(const G &x, const std::vector<K> &init, const rlc_options &o) {
+  return rlc_hip::run(x, &init, o);
+}

@@ -82,8 +82,8 @@ typedef struct rlc_options {
                              chunks, one wavefront per chunk (MG8; 256 lanes per vertex for BM).
                              Non-decreasing; all zero = the library defaults */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
-  int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: vertices are
-                             split by (id / 64) % S into S segments, run one after the other so that
+  int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: blocks of 64
+                             consecutive ids go round-robin into S segments, run one after the other so that
                              later segments see the labels written by earlier ones); 1 = a single
                              sweep. SYNC: ignored */
   int32_t reserved[4];    /* must be zero */

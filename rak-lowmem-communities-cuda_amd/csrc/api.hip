@@ -85,9 +85,14 @@ struct rlc_graph {
   uint32_t *d_lab = nullptr;
   uint8_t *d_aff = nullptr;
   uint64_t *d_moves = nullptr;
+  uint32_t *d_lbuf = nullptr;
   DevCounters *d_ctr = nullptr;
   uint32_t *d_init = nullptr;
   double *d_ctot = nullptr;
+  double *d_K = nullptr;
+  uint32_t *d_mwin = nullptr;
+  uint64_t *d_mstart = nullptr;
+  uint64_t mitems = 0;
   uint8_t *d_seen = nullptr;
   double *d_modout = nullptr;
   bool labels_dense = true;
@@ -104,7 +109,7 @@ struct rlc_graph {
   ChunkPlan plan{};
 
   DevGraph dev() const { return DevGraph{d_off, d_edg, unit ? nullptr : d_wei, n, m}; }
-  DevState state() const { return DevState{d_lab, d_aff, d_moves, d_ctr}; }
+  DevState state() const { return DevState{d_lab, d_lbuf, d_aff, d_moves, d_ctr}; }
 };
 
 namespace {
@@ -125,8 +130,9 @@ void free_graph(rlc_graph *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->hexec) (void)hipGraphExecDestroy(g->hexec);
   if (g->hgraph) (void)hipGraphDestroy(g->hgraph);
-  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_aff, g->d_moves,
-                  g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout, g->d_plan};
+  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_lbuf, g->d_aff, g->d_moves,
+                  g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout, g->d_plan,
+                  g->d_K, g->d_mwin, g->d_mstart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &e : g->ev)
@@ -197,7 +203,7 @@ rlc_status ensure_chunk_plan(rlc_graph *g, uint64_t t) {
   cp.cw = reinterpret_cast<float *>(take(8 * V));
   cp.pc = take(8 * C);
   cp.pw = reinterpret_cast<float *>(take(8 * C));
-  cp.pdec = take(C);
+  cp.pd = reinterpret_cast<float *>(take(C));
   auto up = [&](uint32_t *d, const std::vector<uint32_t> &h) {
     return h.empty() ? hipSuccess : hipMemcpyAsync(d, h.data(), h.size() * 4, hipMemcpyHostToDevice, g->stream);
   };
@@ -491,6 +497,10 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   }
   const bool sync = o->mode == RLC_MODE_SYNC;
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
+  if (o->slots == RLC_SKETCH_MG8 && !g->d_lbuf) {
+    if ((st = dev_alloc(&g->d_lbuf, g->m)) != RLC_OK) return st;
+    g->key.valid = false;  // captured launches embed the state pointers
+  }
   const Schedule cl = build_schedule(g, o, &st);
   if (st != RLC_OK) return st;
 
@@ -577,7 +587,8 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     stats->time_ms = t_total / o->repeat;
     stats->init_ms = t_init / o->repeat;
     stats->marking_ms = t_mark / o->repeat;
-    const uint64_t state_bytes = n * (4 + 1) + (sync ? n * 8 : 0) + sizeof(DevCounters) + (init_labels ? n * 4 : 0);
+    const uint64_t state_bytes = n * (4 + 1) + (sync ? n * 8 : 0) + sizeof(DevCounters) + (init_labels ? n * 4 : 0) +
+                                 (o->slots == RLC_SKETCH_MG8 ? g->m * 4 : 0);
     stats->memory_gb = (float)((double)state_bytes / (1024.0 * 1024.0 * 1024.0));
     stats->changed_last = (uint32_t)hc.last_changed;
     const uint64_t eb = g->unit ? 8 : 12;
@@ -609,9 +620,30 @@ rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels
   const uint64_t n = g->n;
   rlc_status st;
   if ((!g->d_ctot && (st = dev_alloc(&g->d_ctot, n)) != RLC_OK) ||
+      (!g->d_K && (st = dev_alloc(&g->d_K, n)) != RLC_OK) ||
       (!g->d_seen && (st = dev_alloc(&g->d_seen, n)) != RLC_OK) ||
       (!g->d_modout && (st = dev_alloc(&g->d_modout, 4)) != RLC_OK))
     return st;
+  if (!g->d_mwin) {
+    // work items: each 64-position window, its edges cut into pieces of kModPieceEdges
+    std::vector<uint32_t> win;
+    std::vector<uint64_t> start;
+    for (uint64_t p0 = 0; p0 < n; p0 += 64) {
+      const uint64_t eb = g->h_off[p0], ee = g->h_off[std::min<uint64_t>(p0 + 64, n)];
+      for (uint64_t a = eb; a < ee; a += kModPieceEdges) {
+        win.push_back((uint32_t)(p0 / 64));
+        start.push_back(a);
+      }
+    }
+    g->mitems = win.size();
+    if ((st = dev_alloc(&g->d_mwin, g->mitems)) != RLC_OK || (st = dev_alloc(&g->d_mstart, g->mitems)) != RLC_OK)
+      return st;
+    if (g->mitems) {
+      HIP_TRY(hipMemcpyAsync(g->d_mwin, win.data(), g->mitems * 4, hipMemcpyHostToDevice, g->stream));
+      HIP_TRY(hipMemcpyAsync(g->d_mstart, start.data(), g->mitems * 8, hipMemcpyHostToDevice, g->stream));
+      HIP_TRY(hipStreamSynchronize(g->stream));
+    }
+  }
   const uint32_t *dlab = g->d_lab;
   std::vector<uint32_t> dense;
   if (labels || !g->labels_dense) {
@@ -640,7 +672,8 @@ rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels
     if (n) HIP_TRY(hipMemcpyAsync(g->d_init, dense.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, g->stream));
     dlab = g->d_init;
   }
-  launch_modularity(g->dev(), dlab, g->d_ctot, g->d_seen, g->d_modout, g->stream);
+  const ModItems mi{g->d_mwin, g->d_mstart, g->mitems};
+  launch_modularity(g->dev(), dlab, mi, g->d_ctot, g->d_K, g->d_seen, g->d_modout, g->stream);
   if ((st = check_launch()) != RLC_OK) return st;
   double out[4];
   HIP_TRY(hipMemcpyAsync(out, g->d_modout, sizeof(out), hipMemcpyDeviceToHost, g->stream));

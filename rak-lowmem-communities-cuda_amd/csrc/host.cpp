@@ -43,6 +43,8 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
   return RLC_OK;
 }
 
+uint64_t segment_of(uint64_t u, uint64_t S) { return S <= 1 ? 0 : (u / 64) % S; }
+
 rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
                         int segments, Layout &L) {
   const uint64_t S = (uint64_t)(segments < 1 ? 1 : segments);
@@ -59,7 +61,7 @@ rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint3
   for (uint64_t u = 0; u < n; ++u) maxd = std::max<uint64_t>(maxd, off[u + 1] - off[u]);
   // counting sort by (segment, degree descending), stable in id
   const uint64_t D = maxd + 1;
-  auto key = [&](uint64_t u) { return ((u / 64) % S) * D + (maxd - (off[u + 1] - off[u])); };
+  auto key = [&](uint64_t u) { return segment_of(u, S) * D + (maxd - (off[u + 1] - off[u])); };
   std::vector<uint64_t> cnt(S * D + 1, 0);
   for (uint64_t u = 0; u < n; ++u) ++cnt[key(u) + 1];
   for (uint64_t k = 0; k < S * D; ++k) cnt[k + 1] += cnt[k];

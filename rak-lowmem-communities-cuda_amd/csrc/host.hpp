@@ -12,8 +12,13 @@ const char *last_error_cstr();
 rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
                         uint64_t *max_degree);
 
+// Sub-sweep segment of vertex u: its 64-vertex block id, round-robin over S segments, so that
+// a run of consecutive ids (often one community) is spread over all segments and later
+// segments see the labels the earlier ones wrote.
+uint64_t segment_of(uint64_t u, uint64_t S);
+
 // Device layout of a graph: vertices split into `segments` sub-sweep segments by
-// (id / 64) % segments, stored segment after segment, by degree descending inside a segment
+// segment_of(id), stored segment after segment, by degree descending inside a segment
 // (ties: ascending id), so that the most referenced labels share cache lines and every degree
 // class is a contiguous range of each segment. perm[p] = original id of the vertex at position
 // p, inv[u] = position of original vertex u, seg[s] = first position of segment s (seg[S] = n).

@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
+
 // Per-block reduction of the four iteration counters into the device totals.
 struct BlockStats {
   unsigned long long a, e, c, x;
@@ -57,18 +58,20 @@ __device__ __forceinline__ float edge_w(const DevGraph &g, uint64_t i) { return 
 
 constexpr int kBatch = 8;  // edges in flight per lane
 
+
 // Visit adjacency positions i, i+STRIDE, i+2*STRIDE, .. < e of vertex u in order, calling
 // f(label[v], w) for every target v != u. Positions are fetched kBatch at a time (targets and
 // weights, then the label gathers) so that each lane keeps kBatch independent loads in flight;
 // full batches are unpredicated, one predicated batch takes the tail. Indices are 32-bit
 // relative to the first position (a degree is below 2^32).
-template <bool UNIT, int STRIDE, class F>
+template <bool UNIT, int STRIDE, class F, bool STORE = false>
 __device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t i, uint64_t e,
-                                          F f) {
+                                          F f, uint32_t *lbuf = nullptr) {
   if (i >= e) return;
   const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
   const uint32_t *ep = g.edg + i;
   const float *wp = UNIT ? nullptr : g.wei + i;
+  uint32_t *lp = STORE ? lbuf + i : nullptr;
   uint32_t j = 0;
   for (; j + kBatch <= cnt; j += kBatch) {
     uint32_t vv[kBatch], cc[kBatch];
@@ -82,6 +85,10 @@ __device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab
     for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));  // all loads issued before use
+    if (STORE) {
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) lp[(j + k) * STRIDE] = vv[k] != u ? cc[k] : kEmpty;
+    }
 #pragma unroll
     for (int k = 0; k < kBatch; ++k)
       if (vv[k] != u) f(cc[k], ww[k]);
@@ -100,9 +107,42 @@ __device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab
     for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));
+    if (STORE) {
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if ((uint32_t)k < rem) lp[(j + k) * STRIDE] = vv[k] != u ? cc[k] : kEmpty;
+    }
 #pragma unroll
     for (int k = 0; k < kBatch; ++k)
       if ((uint32_t)k < rem && vv[k] != u) f(cc[k], ww[k]);
+  }
+}
+
+// The rescan's view of the same positions: (label, weight) streamed from the label buffer the
+// scan wrote (coalesced reads, no gather); self-loop positions hold kEmpty and are skipped.
+template <bool UNIT, int STRIDE, class F>
+__device__ __forceinline__ void for_labels(const DevGraph &g, const uint32_t *lbuf, uint64_t i, uint64_t e, F f) {
+  if (i >= e) return;
+  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
+  const uint32_t *lp = lbuf + i;
+  const float *wp = UNIT ? nullptr : g.wei + i;
+  uint32_t j = 0;
+  for (; j + kBatch <= cnt; j += kBatch) {
+    uint32_t cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      cc[k] = lp[(j + k) * STRIDE];
+      ww[k] = UNIT ? 1.0f : wp[(j + k) * STRIDE];
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (cc[k] != kEmpty) f(cc[k], ww[k]);
+  }
+  for (; j < cnt; ++j) {
+    const uint32_t c = lp[j * STRIDE];
+    const float w = UNIT ? 1.0f : wp[j * STRIDE];
+    if (c != kEmpty) f(c, w);
   }
 }
 
@@ -173,16 +213,17 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
       if (MG) {
         MG8 sk;
         sk.clear();
-        for_edges<UNIT, 1>(g, s.lab, u, b, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
-        if (sk.dec) {  // rescan (else the slot weights are already exact)
+        auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
+        for_edges<UNIT, 1, decltype(acc_fn), true>(g, s.lab, u, b, e, acc_fn, s.lbuf);
+        if (!certain_argmax(sk.c, sk.w, sk.dsum, deg, cb, wb)) {  // rescan
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) sk.w[j] = 0.0f;
-          for_edges<UNIT, 1>(g, s.lab, u, b, e, [&](uint32_t c, float w) {
+          for_labels<UNIT, 1>(g, s.lbuf, b, e, [&](uint32_t c, float w) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) sk.w[j] = sk.c[j] == c ? sk.w[j] + w : sk.w[j];
           });
+          argmax8(sk.c, sk.w, cb, wb);
         }
-        argmax8(sk.c, sk.w, cb, wb);
       } else {
         uint32_t mc = kEmpty;
         float mw = 0.0f;
@@ -243,8 +284,8 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
   __shared__ float sh_w[GW][kSlots];
   __shared__ uint32_t sh_cand[kSlots];
   __shared__ float sh_tot[kSlots];
-  __shared__ uint32_t sh_flag, sh_flag2;
-  __shared__ uint32_t sh_dec[GW];
+  __shared__ uint32_t sh_flag;
+  __shared__ float sh_d[GW], sh_dt;
   unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
   const uint64_t total = rs.acc[rs.count];
   for (uint64_t base = (uint64_t)blockIdx.x * VPB; base < total; base += (uint64_t)gridDim.x * VPB) {
@@ -274,11 +315,12 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
       if (MG) {
         MG8 sk;
         sk.clear();
-        for_edges<UNIT, P>(g, s.lab, u, b + gl, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
+        auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
+        for_edges<UNIT, P, decltype(acc_fn), true>(g, s.lab, u, b + gl, e, acc_fn, s.lbuf);
         group_merge_tree<W>(sk);
         uint32_t cand[kSlots];
-        float cw[kSlots];  // merged weights: exact when no decrement happened in the group
-        bool any_dec;
+        float cw[kSlots];  // merged weights
+        float dtot;        // decrement weight of the whole group (scans and absorbs)
         if (GW == 1) {
           const int leader = lane & ~(W - 1);
 #pragma unroll
@@ -286,22 +328,23 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
             cand[j] = __shfl(sk.c[j], leader);
             cw[j] = __shfl(sk.w[j], leader);
           }
-          // any lane of the group that decremented, in the scan or in an absorb
-          const unsigned long long dm = __ballot(sk.dec);
-          const unsigned long long gm = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (lane & ~(W - 1) & 63));
-          any_dec = (dm & gm) != 0ull;
+          dtot = sk.dsum;
+#pragma unroll
+          for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
         } else {
-          const bool wdec = __ballot(sk.dec) != 0ull;
+          float wd = sk.dsum;
+#pragma unroll
+          for (int k = 1; k < 64; k <<= 1) wd += __shfl_xor(wd, k);
           if (lane == 0) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) { sh_c[gw][j] = sk.c[j]; sh_w[gw][j] = sk.w[j]; }
-            sh_dec[gw] = wdec ? 1u : 0u;
+            sh_d[gw] = wd;
           }
           __syncthreads();
           if (threadIdx.x == 0) {
             // continue the binary tree across the waves: wave q absorbs wave q+k
-            bool d = false;
-            for (int q = 0; q < GW; ++q) d = d || sh_dec[q] != 0u;
+            float d = 0.0f;
+            for (int q = 0; q < GW; ++q) d += sh_d[q];
             for (int k = 1; k < GW; k <<= 1)
               for (int q = 0; q + k < GW; q += 2 * k) {
                 MG8 a;
@@ -311,28 +354,28 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
                   a.c[j] = sh_c[q][j]; a.w[j] = sh_w[q][j];
                   oc[j] = sh_c[q + k][j]; ow[j] = sh_w[q + k][j];
                 }
-                a.dec = false;
+                a.dsum = 0.0f;
                 a.absorb(oc, ow);
-                d = d || a.dec;
+                d += a.dsum;
                 for (int j = 0; j < kSlots; ++j) { sh_c[q][j] = a.c[j]; sh_w[q][j] = a.w[j]; }
               }
             for (int j = 0; j < kSlots; ++j) { sh_cand[j] = sh_c[0][j]; sh_tot[j] = sh_w[0][j]; }
-            sh_flag2 = d ? 1u : 0u;
+            sh_dt = d;
           }
           __syncthreads();
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) { cand[j] = sh_cand[j]; cw[j] = sh_tot[j]; }
-          any_dec = sh_flag2 != 0u;
+          dtot = sh_dt;
           __syncthreads();
         }
         float acc[kSlots];
-        if (!any_dec) {  // merged weights are the exact tree-ordered sums
+        if (certain_argmax(cand, cw, dtot, deg, cb, wb)) {  // the rescan cannot change the choice
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = cw[j];
         } else {
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
-        for_edges<UNIT, P>(g, s.lab, u, b + gl, e, [&](uint32_t c, float w) {
+        for_labels<UNIT, P>(g, s.lbuf, b + gl, e, [&](uint32_t c, float w) {
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == c ? acc[j] + w : acc[j];
         });
@@ -353,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = sh_tot[j];
         }
-        }
         argmax8(cand, acc, cb, wb);
+        }
       } else {
         uint32_t mc = kEmpty;
         float mw = 0.0f;
@@ -441,18 +484,21 @@ __global__ __launch_bounds__(kBlock) void k_chunk_scan(DevGraph g, DevState s, C
     const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
     MG8 sk;
     sk.clear();
-    for_edges<UNIT, 64>(g, s.lab, u, b + lane, e, [&](uint32_t cl, float w) { sk.accumulate(cl, w); });
+    auto acc_fn = [&](uint32_t cl, float w) { sk.accumulate(cl, w); };
+    for_edges<UNIT, 64, decltype(acc_fn), true>(g, s.lab, u, b + lane, e, acc_fn, s.lbuf);
     group_merge_tree<64>(sk);
-    const bool dec = __ballot(sk.dec) != 0ull;
+    float d = sk.dsum;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) d += __shfl_xor(d, k);
     if (lane == 0) {
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) { cp.pc[c * kSlots + j] = sk.c[j]; cp.pw[c * kSlots + j] = sk.w[j]; }
-      cp.pdec[c] = dec ? 1u : 0u;
+      cp.pd[c] = d;
     }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void k_chunk_merge(DevState s, ChunkPlan cp) {
+__global__ __launch_bounds__(kBlock) void k_chunk_merge(DevGraph g, DevState s, ChunkPlan cp) {
   if (s.ctr->done) return;
   const int lane = lane_id();
   const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -462,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_merge(DevState s, ChunkPlan cp
     const uint32_t f0 = cp.vfirst[slot], nc = cp.vcount[slot];
     MG8 acc;
     acc.clear();
-    bool anyd = false;
+    float d = 0.0f;  // decrement weight: chunk scans + merges
     for (uint32_t r = 0; r < nc; r += 64) {
       MG8 part;
       part.clear();
@@ -471,17 +517,24 @@ __global__ __launch_bounds__(kBlock) void k_chunk_merge(DevState s, ChunkPlan cp
         const uint64_t pcid = (uint64_t)f0 + k;
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) { part.c[j] = cp.pc[pcid * kSlots + j]; part.w[j] = cp.pw[pcid * kSlots + j]; }
-        anyd = anyd || cp.pdec[pcid] != 0u;
+        d += cp.pd[pcid];
       }
       group_merge_tree<64>(part);
       if (lane == 0) acc.absorb(part.c, part.w);
-      anyd = anyd || part.dec;
+      d += part.dsum;
     }
-    anyd = __ballot(anyd || acc.dec) != 0ull;
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) d += __shfl_xor(d, k);
+    d += __shfl(acc.dsum, 0);
     if (lane == 0) {
+      const uint32_t u = cp.vpos[slot];
+      const uint64_t deg = g.off[(uint64_t)u + 1] - g.off[u];
+      uint32_t cb;
+      float wb;
+      const bool sure = certain_argmax(acc.c, acc.w, d, deg, cb, wb);
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) { cp.cand[slot * kSlots + j] = acc.c[j]; cp.cw[slot * kSlots + j] = acc.w[j]; }
-      cp.vdec[slot] = anyd ? 1u : 0u;
+      cp.vdec[slot] = sure ? 0u : 1u;  // rescan needed
     }
   }
 }
@@ -505,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_rescan(DevGraph g, DevState s,
     float acc[kSlots];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
-    for_edges<UNIT, 64>(g, s.lab, u, b + lane, e, [&](uint32_t cl, float w) {
+    for_labels<UNIT, 64>(g, s.lbuf, b + lane, e, [&](uint32_t cl, float w) {
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == cl ? acc[j] + w : acc[j];
     });
@@ -551,7 +604,11 @@ __global__ __launch_bounds__(kBlock) void k_chunk_choose(DevGraph g, DevState s,
     }
     uint32_t cb;
     float wb;
-    argmax8(cand, tot, cb, wb);
+    if (cp.vdec[slot]) {
+      argmax8(cand, tot, cb, wb);
+    } else {  // certified in the merge pass: the first heaviest slot, beyond the others by > D
+      (void)certain_argmax(cand, tot, 0.0f, 0, cb, wb);
+    }
     if (lane == 0) {
       const uint32_t d = s.lab[u];
       const bool moved = wb > 0.0f && cb != d && !(pickless && cb > d);
@@ -660,9 +717,9 @@ __global__ __launch_bounds__(kBlock) void k_mark_all(uint8_t *aff, uint64_t n, D
 }
 
 // ---- FP64 modularity --------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_mod_zero(double *ctot, uint8_t *seen, double *out, uint64_t n) {
+__global__ __launch_bounds__(kBlock) void k_mod_zero(double *ctot, double *K, uint8_t *seen, double *out, uint64_t n) {
   const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  for (uint64_t i = t; i < n; i += (uint64_t)gridDim.x * kBlock) { ctot[i] = 0.0; seen[i] = 0; }
+  for (uint64_t i = t; i < n; i += (uint64_t)gridDim.x * kBlock) { ctot[i] = 0.0; K[i] = 0.0; seen[i] = 0; }
   if (t < 4) out[t] = 0.0;
 }
 
@@ -672,46 +729,100 @@ __device__ __forceinline__ double wave_sum_f64(double x) {
   return x;
 }
 
-// Edge-chunked pass: each thread owns kModChunk consecutive CSR positions, finds the source
-// vertex of its first position by binary search over the offsets, and walks forward.
-// intra += w for same-label endpoints; tot[label[u]] += w (one atomic per source run).
-constexpr uint64_t kModChunk = 64;
-
-__global__ __launch_bounds__(kBlock) void k_mod_scan(DevGraph g, const uint32_t *__restrict__ lab, double *ctot,
-                                                     uint8_t *seen, double *out) {
-  double si = 0.0;
-  const uint64_t nchunks = (g.m + kModChunk - 1) / kModChunk;
-  for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t < nchunks; t += (uint64_t)gridDim.x * kBlock) {
-    const uint64_t i0 = t * kModChunk, i1 = i0 + kModChunk < g.m ? i0 + kModChunk : g.m;
-    // source of position i0: the last u with off[u] <= i0 (and off[u+1] > i0)
-    uint64_t lo = 0, hi = g.n;  // invariant: off[lo] <= i0 < off[hi]
-    while (hi - lo > 1) {
-      const uint64_t mid = (lo + hi) >> 1;
-      if (g.off[mid] <= i0) lo = mid;
-      else hi = mid;
-    }
-    uint64_t u = lo, uend = g.off[u + 1];
-    uint32_t c = lab[u];
-    double run = 0.0;
-    for (uint64_t i = i0; i < i1; ++i) {
-      while (i >= uend) {
-        if (run != 0.0) atomicAdd(&ctot[c], run);
-        run = 0.0;
-        ++u;
-        uend = g.off[u + 1];
-        c = lab[u];
-      }
-      const double w = (double)edge_w(g, i);
-      run += w;
-      if (lab[g.edg[i]] == c) si += w;
-    }
-    if (run != 0.0) atomicAdd(&ctot[c], run);
+// tot[c] += x, combining lanes of the wave that hold the same label into one atomic (label
+// propagation leaves few, large communities, and their accumulators would serialise).
+__device__ __forceinline__ void add_tot(double *ctot, uint32_t c, double x, bool want) {
+  unsigned long long pending = __ballot(want);
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const uint32_t cl = __shfl(c, leader);
+    const bool mine = want && c == cl;
+    const unsigned long long group = __ballot(mine);
+    double v = mine ? x : 0.0;
+#pragma unroll
+    for (int k = 32; k >= 1; k >>= 1) v += __shfl_xor(v, k);
+    if (lane_id() == leader && v != 0.0) atomicAdd(&ctot[cl], v);
+    pending &= ~group;
+    want = want && !mine;
   }
-  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < g.n; u += (uint64_t)gridDim.x * kBlock) {
-    seen[lab[u]] = 1;
+}
+
+// Work item = (window of 64 consecutive vertices, piece of at most kModPieceEdges of the window's
+// contiguous edges). One wave per item walks its piece 64 positions at a time (coalesced); each
+// lane keeps the window index of the source of its current position (monotone, advanced over
+// the window's offsets in LDS) and flushes the source's running weight to K[u] when the source
+// changes. intra += w for same-label endpoints.
+__global__ __launch_bounds__(kBlock) void k_mod_edges(DevGraph g, const uint32_t *__restrict__ lab,
+                                                      const uint32_t *__restrict__ iwin,
+                                                      const uint64_t *__restrict__ istart, uint64_t nitems,
+                                                      double *K, double *out) {
+  __shared__ uint64_t s_off[kBlock / 64][65];
+  __shared__ uint32_t s_lab[kBlock / 64][64];
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  double si = 0.0;
+  const uint64_t wg = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6, nw = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t it = wg; it < nitems; it += nw) {
+    const uint64_t p0 = (uint64_t)iwin[it] * 64;
+    const uint64_t u = p0 + (uint64_t)lane;
+    const bool valid = u < g.n;
+    s_off[wave][lane] = g.off[valid ? u : g.n];
+    if (lane == 0) s_off[wave][64] = g.off[p0 + 64 < g.n ? p0 + 64 : g.n];
+    s_lab[wave][lane] = valid ? lab[u] : 0u;
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t a = istart[it];
+    const uint64_t ee = s_off[wave][64];
+    const uint64_t b = a + kModPieceEdges < ee ? a + kModPieceEdges : ee;
+    int own = 0;
+    double run = 0.0;
+    constexpr int kU = 4;
+    for (uint64_t i0 = a + (uint64_t)lane; i0 < b; i0 += 64 * kU) {
+      uint32_t vv[kU], cc[kU];
+      float ww[kU];
+#pragma unroll
+      for (int k = 0; k < kU; ++k) {
+        const uint64_t i = i0 + 64 * (uint64_t)k;
+        const uint64_t ii = i < b ? i : i0;
+        vv[k] = g.edg[ii];
+        ww[k] = edge_w(g, ii);
+      }
+#pragma unroll
+      for (int k = 0; k < kU; ++k) cc[k] = lab[vv[k]];
+#pragma unroll
+      for (int k = 0; k < kU; ++k) {
+        const uint64_t i = i0 + 64 * (uint64_t)k;
+        if (i >= b) break;
+        int o = own;
+        while (s_off[wave][o + 1] <= i) ++o;
+        if (o != own) {
+          if (run != 0.0) atomicAdd(&K[p0 + own], run);
+          run = 0.0;
+          own = o;
+        }
+        const double w = (double)ww[k];
+        run += w;
+        if (cc[k] == s_lab[wave][own]) si += w;
+      }
+    }
+    if (run != 0.0) atomicAdd(&K[p0 + own], run);
+    __builtin_amdgcn_wave_barrier();
   }
   si = wave_sum_f64(si);
-  if (lane_id() == 0) atomicAdd(&out[0], si);
+  if (lane == 0) atomicAdd(&out[0], si);
+}
+
+// tot[label[u]] += K[u] (equal labels of a wave combined), seen[label[u]] = 1
+__global__ __launch_bounds__(kBlock) void k_mod_vertex(DevGraph g, const uint32_t *__restrict__ lab, const double *K,
+                                                       double *ctot, uint8_t *seen) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t rounds = (g.n + stride - 1) / stride;
+  for (uint64_t r = 0; r < rounds; ++r) {  // wave-uniform trip count (add_tot is collective)
+    const uint64_t u = r * stride + (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool valid = u < g.n;
+    const uint32_t c = valid ? lab[u] : 0u;
+    const double k = valid ? K[u] : 0.0;
+    if (valid) seen[c] = 1;
+    add_tot(ctot, c, k, valid && k != 0.0);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_mod_sum(const double *ctot, const uint8_t *seen, uint64_t n, double *out) {
@@ -758,7 +869,7 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     const unsigned gr = grid_for(sc.plan.nchunks, kBlock / 64, kGridCap);
     const unsigned gv = grid_for(sc.plan.nslots, kBlock / 64, kGridCap);
     k_chunk_scan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
-    k_chunk_merge<<<gv, kBlock, 0, st>>>(s, sc.plan);
+    k_chunk_merge<<<gv, kBlock, 0, st>>>(g, s, sc.plan);
     k_chunk_rescan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
     k_chunk_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
     k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
@@ -813,10 +924,12 @@ void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, 
   k_finish<<<1, 64, 0, st>>>(s.ctr, g.n, p.tolerance, p.max_iterations, p.pickstep, p.sync ? 1 : 0);
 }
 
-void launch_modularity(const DevGraph &g, const uint32_t *lab, double *ctot, uint8_t *seen, double *out,
-                       hipStream_t st) {
-  k_mod_zero<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(ctot, seen, out, g.n);
-  k_mod_scan<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(g, lab, ctot, seen, out);
+void launch_modularity(const DevGraph &g, const uint32_t *lab, const ModItems &mi, double *ctot, double *K,
+                       uint8_t *seen, double *out, hipStream_t st) {
+  k_mod_zero<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(ctot, K, seen, out, g.n);
+  if (mi.count) k_mod_edges<<<grid_for(mi.count, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, lab, mi.win, mi.start,
+                                                                                        mi.count, K, out);
+  k_mod_vertex<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(g, lab, K, ctot, seen);
   k_mod_sum<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(ctot, seen, g.n, out);
 }
 

@@ -30,6 +30,8 @@ struct DevCounters {
 
 struct DevState {
   uint32_t *lab;     // labels (community of each vertex)
+  uint32_t *lbuf;    // per edge position: the target's label read by the scan (0xFFFFFFFF for a
+                     // self-loop), streamed back by the rescan instead of gathering again
   uint8_t *aff;      // vertex-pruning flags: 1 = process in the next iteration
   uint64_t *moves;   // synchronous mode change list: (u << 32) | new label
   DevCounters *ctr;
@@ -66,12 +68,12 @@ struct ChunkPlan {
   uint32_t *cnt;           // slot -> chunks finished in the current pass (reset by the last)
   uint32_t *act;           // slot -> vertex processed in this iteration
   uint32_t *moved;         // slot -> vertex changed label in this iteration
-  uint32_t *vdec;          // slot -> a decrement happened somewhere (rescan needed)
+  uint32_t *vdec;          // slot -> the rescan pass is needed (choice not certain from the sketch)
   uint32_t *cand;          // slot * 8 -> merged sketch labels
   float *cw;               // slot * 8 -> merged sketch weights
   uint32_t *pc;            // chunk * 8 -> partial sketch labels
   float *pw;               // chunk * 8 -> partial sketch weights / rescan partial sums
-  uint32_t *pdec;          // chunk -> partial sketch decremented
+  float *pd;               // chunk -> decrement weight of the chunk's scan and wave merge
 };
 
 // The move launches of one iteration, in order (chunked vertices first when present).
@@ -98,10 +100,19 @@ void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, h
 // the finish step. ev_move0/ev_move1 (may be null) bracket the move launches.
 void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
                       hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1);
+// Work items of the modularity pass: (window of 64 consecutive positions, first edge of a piece
+// of at most 4096 of the window's edges).
+struct ModItems {
+  const uint32_t *win;
+  const uint64_t *start;
+  uint64_t count;
+};
+constexpr uint64_t kModPieceEdges = 4096;
+
 // FP64 modularity terms: out[0] = sum of intra-community weight, out[1] = total weight,
 // out[2] = sum over labels of tot^2, out[3] = number of distinct labels. Needs labels < n.
-// scratch: n doubles + n bytes.
-void launch_modularity(const DevGraph &g, const uint32_t *lab, double *ctot, uint8_t *seen, double *out,
-                       hipStream_t st);
+// scratch: 2n doubles + n bytes.
+void launch_modularity(const DevGraph &g, const uint32_t *lab, const ModItems &mi, double *ctot, double *K,
+                       uint8_t *seen, double *out, hipStream_t st);
 
 }  // namespace rlc

@@ -17,20 +17,20 @@ constexpr int kSlots = 8;
 // zero-weight slot; otherwise subtract w from every slot, clamped at zero. (The reference GPU
 // variant, inc/sketchCuda.hxx:73-108, takes the lowest free slot and does not clamp.)
 //
-// `dec` records whether any decrement happened. Without one, no label was ever evicted and each
-// slot's weight is the sum of its label's weights in scan order, starting from the insert,
-// which is bit for bit what the reference's rescan (inc/rakLowmem.hxx:154-162) recomputes; the
-// rescan pass is then skipped. The same holds across a merge tree whose absorbs never decrement:
-// absorb adds part sums in the order the tree-ordered rescan adds them.
+// `dsum` is the total weight of the decrement events. Misra-Gries guarantees, for every slot
+// label c, exact(c) - w(c) <= dsum (also across a merge tree, with the absorbs' decrements
+// added), where exact(c) is what the reference's rescan (inc/rakLowmem.hxx:154-162) computes.
+// So the rescan is skipped when it cannot change the choice (certain_argmax below); with
+// dsum == 0 the slot weights already equal the rescan sums bit for bit (same adds, same order).
 struct MG8 {
   uint32_t c[kSlots];
   float w[kSlots];
-  bool dec;
+  float dsum;
 
   __device__ __forceinline__ void clear() {
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) { c[i] = kEmpty; w[i] = 0.0f; }
-    dec = false;
+    dsum = 0.0f;
   }
 
   __device__ __forceinline__ void accumulate(uint32_t lab, float wt) {
@@ -50,7 +50,7 @@ struct MG8 {
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) f = (w[i] == 0.0f) ? i : f;
     const bool full = f < 0;
-    dec = dec || full;
+    dsum = full ? dsum + wt : dsum;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
       const bool sel = i == f;
@@ -76,6 +76,30 @@ __device__ __forceinline__ void argmax8(const uint32_t (&c)[kSlots], const float
 #pragma unroll
   for (int i = 0; i < kSlots; ++i)
     if (w[i] > wb) { wb = w[i]; cb = c[i]; }
+}
+
+// The reference's choice after a rescan, when it is already determined by the sketch: with no
+// decrement (D == 0) the weights are exact; otherwise the first heaviest slot must beat every
+// other slot by more than D plus a rounding margin (the rescan sums lie in [w, w + D]). Returns
+// false when a rescan is needed. deg bounds the number of float additions behind each weight.
+__device__ __forceinline__ bool certain_argmax(const uint32_t (&c)[kSlots], const float (&w)[kSlots], float D,
+                                               uint64_t deg, uint32_t &cb, float &wb) {
+  if (D == 0.0f) {
+    argmax8(c, w, cb, wb);
+    return true;
+  }
+  int top = 0;
+#pragma unroll
+  for (int i = 1; i < kSlots; ++i) top = w[i] > w[top] ? i : top;
+  float second = 0.0f;
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i) second = (i != top && w[i] > second) ? w[i] : second;
+  const float eps = (float)(deg + 8) * 0x1.0p-21f * (w[top] + D);
+  if (!(w[top] > second + D + eps) || !(w[top] > eps)) return false;
+#pragma unroll
+  for (int i = 0; i < kSlots; ++i)
+    if (i == top) { cb = c[i]; wb = w[i]; }
+  return true;
 }
 
 // Weighted Boyer-Moore majority step (inc/rakLowmem.hxx:121-125, inc/rakLowmemCuda.hxx:49-51).

@@ -1,0 +1,84 @@
+// Uses the C++ drop-in header the way the reference's driver uses rakLowmemStaticCuda
+// (main.cxx:84-92): a graph type with the reference's graph concept, SLOTS=8 and SLOTS=1, plus
+// rakLowmemCuda with an initial membership. Vertex 0 is absent (ids are 1-based, as after
+// reading a Matrix-Market file, inc/mtx.hxx:213). Exit status 0 when every check holds.
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <set>
+#include <utility>
+#include <vector>
+
+#include "rak_lowmem_hip.hxx"
+
+// Minimal graph with the concept the header consumes (inc/Graph.hxx:59-168).
+struct TinyGraph {
+  using key_type = uint32_t;
+  using edge_value_type = float;
+  std::vector<bool> exists;
+  std::vector<std::vector<std::pair<uint32_t, float>>> adj;  // sorted by target
+  size_t span() const { return exists.size(); }
+  size_t order() const { size_t n = 0; for (bool e : exists) n += e; return n; }
+  bool hasVertex(uint32_t u) const { return u < span() && exists[u]; }
+  template <class F> void forEachVertexKey(F f) const { for (uint32_t u = 0; u < span(); ++u) if (exists[u]) f(u); }
+  template <class F> void forEachEdge(uint32_t u, F f) const { for (auto &p : adj[u]) f(p.first, p.second); }
+  void addEdge(uint32_t u, uint32_t v, float w = 1) {
+    uint32_t m = std::max(u, v) + 1;
+    if (exists.size() < m) { exists.resize(m); adj.resize(m); }
+    exists[u] = exists[v] = true;
+    adj[u].push_back({v, w});
+    adj[v].push_back({u, w});
+  }
+  void finish() { for (auto &a : adj) std::sort(a.begin(), a.end()); }
+};
+
+int main() {
+  TinyGraph x;
+  // three disjoint cliques on ids 1..6, 7..15, 16..27 (vertex 0 absent)
+  uint32_t b = 1;
+  for (uint32_t s : {6u, 9u, 12u}) {
+    for (uint32_t i = 0; i < s; ++i)
+      for (uint32_t j = i + 1; j < s; ++j) x.addEdge(b + i, b + j);
+    b += s;
+  }
+  x.finish();
+  int bad = 0;
+  auto check_cliques = [&](const std::vector<uint32_t> &m, const char *what) {
+    if (m.size() != x.span()) { std::printf("%s: membership size %zu != span %zu\n", what, m.size(), x.span()); ++bad; }
+    if (m[0] != 0) { std::printf("%s: absent vertex 0 has label %u\n", what, m[0]); ++bad; }
+    uint32_t start = 1;
+    std::set<uint32_t> seen;
+    for (uint32_t s : {6u, 9u, 12u}) {
+      std::set<uint32_t> labs(m.begin() + start, m.begin() + start + s);
+      if (labs.size() != 1) { std::printf("%s: clique at %u has %zu labels\n", what, start, labs.size()); ++bad; }
+      uint32_t l = *labs.begin();
+      if (!x.hasVertex(l) || l < start || l >= start + s) { std::printf("%s: label %u not a member id\n", what, l); ++bad; }
+      seen.insert(l);
+      start += s;
+    }
+    if (seen.size() != 3) { std::printf("%s: cliques share labels\n", what); ++bad; }
+  };
+  auto a = rakLowmemStaticCuda<8>(x, RakOptions(1, 0.0, 20));
+  check_cliques(a.membership, "MG8");
+  auto c = rakLowmemStaticCuda<1>(x, RakOptions(2, 0.0, 20));
+  check_cliques(c.membership, "BM");
+  // resume from an initial membership: every vertex already labelled by its clique -> no moves;
+  // iteration 0 is pick-less and skips the convergence test (rakLowmemCuda.hxx:476, :487), so
+  // the run stops after iteration 1
+  std::vector<uint32_t> init(x.span(), 0);
+  uint32_t start = 1, tag = 1000;
+  for (uint32_t s : {6u, 9u, 12u}) { for (uint32_t i = 0; i < s; ++i) init[start + i] = tag; tag += 7; start += s; }
+  auto d = rakLowmemCuda<8>(x, init, RakOptions());
+  for (uint32_t u = 1; u < x.span(); ++u)
+    if (d.membership[u] != init[u]) { std::printf("init: vertex %u moved to %u\n", u, d.membership[u]); ++bad; break; }
+  if (d.iterations != 2) { std::printf("init: %d iterations, expected 2\n", d.iterations); ++bad; }
+  // deterministic mode through the full options
+  rlc_options o;
+  rlc_options_default(&o);
+  o.mode = RLC_MODE_SYNC;
+  auto e = rakLowmemStaticCuda(x, o);
+  check_cliques(e.membership, "SYNC");
+  std::printf("drop-in: MG8 %d iterations %.3f ms, BM %d iterations, SYNC %d iterations; %s\n", a.iterations, a.time,
+              c.iterations, e.iterations, bad ? "FAILED" : "ok");
+  return bad ? 1 : 0;
+}

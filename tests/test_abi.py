@@ -1,0 +1,79 @@
+"""The C ABI library: loads without a GPU, exports every entry point include/rlc.h declares, and
+reports errors as status codes (no exit, no abort) when no device is visible. No compute calls."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import rlc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    src = open(os.path.join(ROOT, "include", "rlc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rlc_\w+)\s*\(", src)))
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(rlc.LIB_PATH)
+    names = declared()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares a signature for each of them
+    bound = {n for n, _, _ in rlc.SIGNATURES}
+    assert set(names) <= bound, set(names) - bound
+
+
+def test_status_strings_and_defaults():
+    L = rlc.lib()
+    assert L.rlc_abi_version() == 1
+    assert L.rlc_status_string(0) == b"ok"
+    o = rlc.options()
+    assert (o.repeat, o.tolerance, o.max_iterations, o.slots, o.mode, o.pickstep) == (1, 0.05, 20, 8, 0, 8)
+    assert list(o.group_degree) == [0, 0, 0, 0] and list(o.reserved) == [0, 0, 0, 0]
+
+
+def test_errors_are_status_codes():
+    # no HIP device in a CPU-only environment: a clean status, never an exit
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(rlc.RlcError) as e:
+        rlc.Context(0)
+    assert e.value.status in (5, 4)
+    # null out-pointers are rejected
+    assert rlc.lib().rlc_context_create(0, None) == 1
+    assert b"null" in rlc.lib().rlc_last_error()
+
+
+def test_generators_are_deterministic_and_symmetric():
+    import numpy as np
+    import scipy.sparse as sp
+
+    for g in (rlc.gen_sbm(3000, 12, 9.0, 0.8, 4), rlc.gen_rmat(11, 8, wmin=0.5, wmax=1.5, seed=2),
+              rlc.gen_grid(40, 50, 0.1, 0.02, 3)):
+        A = sp.csr_matrix((g.weights if g.weights is not None else np.ones(g.m), g.edges, g.offsets),
+                          shape=(g.n, g.n))
+        assert (A != A.T).nnz == 0  # symmetric, symmetric weights
+        assert A.diagonal().sum() == 0  # no self-loops
+        for u in range(0, g.n, 97):  # sorted, no duplicates
+            row = g.edges[g.offsets[u]:g.offsets[u + 1]]
+            assert np.all(np.diff(row.astype(np.int64)) > 0)
+    a, b = rlc.gen_rmat(10, 16, seed=9), rlc.gen_rmat(10, 16, seed=9)
+    assert np.array_equal(a.offsets, b.offsets) and np.array_equal(a.edges, b.edges)
+    g = rlc.gen_sbm(65536, 256, 16.0, 0.8, 1)
+    assert abs(g.m / g.n - 16.0) < 0.2  # expected degree 16
+    blk = np.repeat(np.arange(g.n) // 256, np.diff(g.offsets).astype(np.int64))
+    intra = np.mean(blk == g.edges // 256)
+    assert abs(intra - 0.8) < 0.01  # 80% of edges inside blocks
+
+
+def test_dropin_header_compiles():
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_test")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
