@@ -35,6 +35,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(nbytes):
+    """One hipMemcpy device-to-device of nbytes: the copy kernel whose counters read a known size."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: {nbytes} bytes device to device")
+
+
+def load_traffic(args):
+    """HBM bytes per move-phase iteration measured by rocprofv3 counters on this workload (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "rmat24_mg8_async_traffic.json")
+    if args.scale != 24 or args.slots != 8 or args.group_degree != "0,0,0,0" or args.sweeps != 0:
+        return None
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return {"bytes_per_iteration": t["traffic_bytes_per_iteration"], "source": os.path.relpath(path, ROOT)}
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +74,9 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="after timing, run one device-to-device copy of this many bytes (the known-size kernel "
+                         "that scales FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -116,6 +147,12 @@ def main():
     }
     dg.close()
     ctx.close()
+    if args.calibrate_copy:
+        copy_probe(args.calibrate_copy)
+    traffic = load_traffic(args)
+    if traffic is not None:
+        line["roofline"]["traffic"] = traffic["bytes_per_iteration"]
+        line["roofline"]["traffic_source"] = traffic["source"]
     if not args.no_cpu_baseline:
         import oracle
 

@@ -86,7 +86,11 @@ typedef struct rlc_options {
                              consecutive ids go round-robin into S segments, run one after the other so that
                              later segments see the labels written by earlier ones); 1 = a single
                              sweep. SYNC: ignored */
-  int32_t reserved[4];    /* must be zero */
+  int32_t no_rescan;      /* ASYNC, MG8: 1 = choose from the sketch weights without a rescan, as the
+                             reference GPU kernels do (RESCAN=false, rakLowmemCuda.hxx:469, :624);
+                             0 = rescan whenever the sketch cannot certify the choice (the CPU
+                             reference's RESCAN=true result) [0]. SYNC: must be 0 */
+  int32_t reserved[3];    /* must be zero */
 } rlc_options;
 
 typedef struct rlc_run_stats {

@@ -44,13 +44,13 @@ int segments_for(uint64_t n) {
 // Everything a captured iteration loop depends on besides the graph itself.
 struct ExecKey {
   bool valid = false;
-  int sync, mg, pickstep, max_iterations, merge_degree, merge_parts, sweeps;
+  int sync, mg, pickstep, max_iterations, merge_degree, merge_parts, sweeps, no_rescan;
   int group_degree[4];
   double tolerance;
   bool operator==(const ExecKey &o) const {
     return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
            max_iterations == o.max_iterations && merge_degree == o.merge_degree && merge_parts == o.merge_parts &&
-           sweeps == o.sweeps && tolerance == o.tolerance &&
+           sweeps == o.sweeps && no_rescan == o.no_rescan && tolerance == o.tolerance &&
            std::equal(group_degree, group_degree + 4, o.group_degree);
   }
 };
@@ -302,6 +302,9 @@ rlc_status check_options(const rlc_options *o) {
     if (k && o->group_degree[k] < o->group_degree[k - 1])
       return fail(RLC_ERR_INVALID_ARGUMENT, "options.group_degree must be non-decreasing");
   }
+  if (o->no_rescan != 0 && o->no_rescan != 1) return fail(RLC_ERR_INVALID_ARGUMENT, "options.no_rescan must be 0 or 1");
+  if (o->no_rescan && o->mode == RLC_MODE_SYNC)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.no_rescan is for the asynchronous mode only");
   if (o->sweeps != 0 && o->sweeps != 1)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.sweeps must be 0 (the graph's segments) or 1");
   if (o->timing != RLC_TIMING_GRAPH && o->timing != RLC_TIMING_PER_ITERATION)
@@ -497,7 +500,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   }
   const bool sync = o->mode == RLC_MODE_SYNC;
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
-  if (o->slots == RLC_SKETCH_MG8 && !g->d_lbuf) {
+  if (o->slots == RLC_SKETCH_MG8 && !o->no_rescan && !g->d_lbuf) {
     if ((st = dev_alloc(&g->d_lbuf, g->m)) != RLC_OK) return st;
     g->key.valid = false;  // captured launches embed the state pointers
   }
@@ -506,7 +509,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
 
   const DevGraph dg = g->dev();
   const DevState ds = g->state();
-  LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, o->pickstep, o->tolerance, o->max_iterations};
+  LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, !o->no_rescan, o->pickstep, o->tolerance, o->max_iterations};
   const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
 
   if (graph_mode && o->max_iterations > 0) {
@@ -520,6 +523,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     k.merge_degree = sync ? o->merge_degree : 0;
     k.merge_parts = sync ? o->merge_parts : 0;
     k.sweeps = sync ? 0 : o->sweeps;
+    k.no_rescan = o->no_rescan;
     for (int q = 0; q < 4; ++q) k.group_degree[q] = sync ? 0 : o->group_degree[q];
     if (!(k == g->key)) {
       if (g->hexec) { (void)hipGraphExecDestroy(g->hexec); g->hexec = nullptr; }
@@ -588,7 +592,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     stats->init_ms = t_init / o->repeat;
     stats->marking_ms = t_mark / o->repeat;
     const uint64_t state_bytes = n * (4 + 1) + (sync ? n * 8 : 0) + sizeof(DevCounters) + (init_labels ? n * 4 : 0) +
-                                 (o->slots == RLC_SKETCH_MG8 ? g->m * 4 : 0);
+                                 (o->slots == RLC_SKETCH_MG8 && !o->no_rescan ? g->m * 4 : 0);
     stats->memory_gb = (float)((double)state_bytes / (1024.0 * 1024.0 * 1024.0));
     stats->changed_last = (uint32_t)hc.last_changed;
     const uint64_t eb = g->unit ? 8 : 12;

@@ -86,6 +86,7 @@ struct Schedule {
 struct LoopParams {
   bool sync;
   bool mg;          // MG8 (true) or BM (false)
+  bool rescan;      // MG8: rescan uncertified choices (false: choose from the sketch weights)
   int pickstep;
   double tolerance;
   int max_iterations;
