@@ -63,6 +63,140 @@ def load_traffic(args):
         return None
 
 
+def bench_batch(args):
+    """Batched workload: seeded SBM graphs (vertex counts log-uniform in [1k, 1M], 4..1024 blocks,
+    degree U[4, 32], 70-90% intra-block edges, unit weights) packed into one device arena and run
+    in the deterministic synchronous mode (MG8), all graphs at once vs one graph per launch."""
+    import hashlib
+
+    import numpy as np
+
+    import rlc
+
+    P = rlc.sbm_mix_params(args.graphs, seed=args.batch_seed)
+    nsum = int(P[0].sum())
+    ecap = int(1.15 * float((P[0].astype(np.float64) * P[2]).sum())) + (1 << 20)
+    ctx = rlc.Context(0)
+    b = ctx.batch(nsum + args.graphs, ecap, args.graphs)
+    t0 = time.time()
+    chunk = 256
+    m_total = 0
+    sample = {}  # a few graphs kept on the host for the check against rlc_run
+    pick = set(np.random.default_rng(7).choice(args.graphs, size=min(8, args.graphs), replace=False).tolist())
+    pick |= {int(np.argmin(P[0])), int(np.argmax(P[0]))}
+    for c0 in range(0, args.graphs, chunk):
+        sl = slice(c0, min(c0 + chunk, args.graphs))
+        S = rlc.gen_sbm_set(*(x[sl] for x in P))
+        for i in range(len(S)):
+            g = S.graph(i)
+            gid = b.add(g)
+            assert gid == c0 + i
+            m_total += g.m
+            if gid in pick:
+                sample[gid] = rlc.Graph(g.offsets.copy(), g.edges.copy(), None)
+        log(f"batch: {min(c0 + chunk, args.graphs)}/{args.graphs} graphs generated and added "
+            f"({time.time() - t0:.0f}s, {m_total} entries)")
+    opts = rlc.options(mode=rlc.MODE_SYNC, slots=args.slots)
+    t1 = time.time()
+    tuned = b.tune(opts)
+    log(f"autotuner: {tuned.as_dict()} ({time.time() - t1:.1f}s)")
+    for _ in range(args.warmup):
+        b.run(opts)
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        st = b.run(opts)
+    wall_b = time.perf_counter() - t_start
+    log(f"batched: {args.steps} runs, {wall_b:.3f}s, steps={st.steps}, device {st.time_ms:.1f} ms/run")
+    digest = []
+    for gid in range(args.graphs):
+        lab, it = b.result(gid)
+        digest.append((hashlib.blake2b(lab.tobytes(), digest_size=16).hexdigest(), it))
+    # reference: each sampled graph alone through rlc_run (its own upload, deterministic mode)
+    for gid, g in sample.items():
+        with ctx.upload(g) as dg:
+            r = dg.run(opts)
+        lab, it = b.result(gid)
+        assert it == r.iterations and np.array_equal(lab, r.membership), f"graph {gid} differs from rlc_run"
+    # one launch per graph, with a tuning of its own (autotuned on the largest graph alone)
+    big = int(np.argmax(P[0]))
+    tuned_1 = b.tune(opts, ids=[big])
+    log(f"autotuner, one graph per launch: {tuned_1.as_dict()}")
+    t_start = time.perf_counter()
+    for gid in range(args.graphs):
+        b.run(opts, ids=[gid], tune=tuned_1)
+    wall_1 = time.perf_counter() - t_start
+    # and the single-graph entry point on a random sample: rlc_run on each graph uploaded alone
+    # (upload and label copy-back outside the timed call)
+    t_run = 0.0
+    rsample = sorted(np.random.default_rng(11).choice(args.graphs, size=min(args.run_sample, args.graphs),
+                                                      replace=False).tolist())
+    for c0 in range(0, len(rsample), 64):
+        part = rsample[c0:c0 + 64]
+        S = rlc.gen_sbm_set(*(x[part] for x in P))
+        for i, gid in enumerate(part):
+            with ctx.upload(S.graph(i)) as dg:
+                dg.run(opts, labels=False)  # warm: graph capture
+                t = time.perf_counter()
+                dg.run(opts, labels=False)
+                t_run += time.perf_counter() - t
+    run_rate = len(rsample) / t_run if t_run > 0 else 0.0
+    log(f"rlc_run on {len(rsample)} sampled graphs: {t_run:.3f}s ({run_rate:.1f} graphs/s)")
+    same = 0
+    for gid in range(args.graphs):
+        lab, it = b.result(gid)
+        same += (hashlib.blake2b(lab.tobytes(), digest_size=16).hexdigest(), it) == digest[gid]
+    log(f"one-by-one: {wall_1:.3f}s; identical results {same}/{args.graphs}")
+    value = args.graphs * args.steps / wall_b
+    one = args.graphs / wall_1
+    line = {
+        "metric": "batched vs 1-by-1 graphs/s",
+        "value": value,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall_b * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.graphs} seeded SBM graphs, n log-uniform [1e3, 1e6], 4-1024 blocks, degree U[4,32], "
+                        f"intra U[0.7,0.9], unit weights, one CSR arena, {'MG8' if args.slots == 8 else 'BM'} "
+                        f"deterministic synchronous mode, tol 0.05, max 20 iterations",
+            "vertices": nsum, "entries": m_total, "tuning": tuned.as_dict(),
+            "one_by_one_graphs_per_s": one, "one_by_one_tuning": tuned_1.as_dict(),
+            "rlc_run_graphs_per_s": run_rate, "rlc_run_sample": len(rsample),
+            "speedup_vs_one_by_one": value / max(one, run_rate),
+            "identical_labels": f"{same}/{args.graphs}", "batch_steps": st.steps,
+            "traversed_edges_per_run": st.traversed_edges,
+            "rlc_run_checked_graphs": sorted(sample),
+        },
+    }
+    b.close()
+    ctx.close()
+    if not args.no_cpu_baseline:
+        import oracle
+
+        # the oracle's deterministic mode, graph after graph, until about 10 s of CPU work
+        cores = len(os.sched_getaffinity(0))
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        t_cpu, k = 0.0, 0
+        while k < args.graphs and t_cpu < 10.0:
+            g = rlc.gen_sbm(int(P[0][k]), int(P[1][k]), float(P[2][k]), float(P[3][k]), int(P[4][k]))
+            t = time.perf_counter()
+            oracle.lpa_sync(g, slots=args.slots, threads=cores)
+            t_cpu += time.perf_counter() - t
+            k += 1
+        line["cpu_baseline"] = {
+            "value": k / t_cpu, "unit": "graphs/s", "cores": cores, "kind": "port",
+            "sample": f"the first {k} graphs of the mix (random order of the mix) through the oracle's "
+                      f"deterministic synchronous mode, {cores} threads, {t_cpu:.1f} s",
+        }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +208,11 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--workload", choices=["rmat", "batch"], default="rmat")
+    ap.add_argument("--graphs", type=int, default=4096, help="batch workload: number of graphs")
+    ap.add_argument("--batch-seed", type=int, default=1)
+    ap.add_argument("--run-sample", type=int, default=256,
+                    help="batch workload: graphs timed through rlc_run one at a time")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many bytes (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run)")
@@ -82,6 +221,10 @@ def main():
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        bench_batch(args)
+        return
 
     import numpy as np
 

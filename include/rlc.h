@@ -171,6 +171,75 @@ rlc_status rlc_graph_vertex_order(const rlc_graph *g, uint32_t *order);
 rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels, double *modularity,
                           uint64_t *communities);
 
+/* ---- batches: many graphs resident in one CSR arena ---------------------------------------
+ * No counterpart in the reference, which runs one graph per call. Graphs are added to a device
+ * arena once; rlc_batch_run advances any subset of them together, one label-propagation
+ * iteration per step, with a persistent kernel that takes 64-vertex work items of every running
+ * graph from a device-side queue. Each graph keeps its own iteration count and convergence test,
+ * so in RLC_MODE_SYNC its labels equal those of rlc_run on the graph alone, bit for bit. The
+ * step loop is captured as a hipGraph (one host synchronisation per run). */
+typedef struct rlc_batch rlc_batch;
+
+/* Execution parameters of a batch run (results of RLC_MODE_SYNC do not depend on them). */
+typedef struct rlc_batch_tuning {
+  int32_t blocks_per_cu; /* persistent grid: workgroups of 256 lanes per compute unit, 1..16 [4] */
+  int32_t grab;          /* 64-vertex work items taken per queue pop, 1..256 [4] */
+  int32_t group_degree;  /* ASYNC: vertices of degree >= group_degree are scanned by group_lanes lanes
+                            (0 = never) [32] */
+  int32_t group_lanes;   /* ASYNC: 4 or 16 [4] */
+  int32_t wave_degree;   /* ASYNC: vertices of degree >= wave_degree are scanned by a whole wavefront
+                            (0 = never) [256]. SYNC: the classes follow options.merge_degree and
+                            merge_parts (4, 16 or 64) exactly as in rlc_run */
+  int32_t interleave;    /* ASYNC: vertex sets of the 8 sub-sweeps of a 512-vertex row: 0 = 64
+                            consecutive vertices, 1 = every 8th vertex, 2 = 8 vertices of each
+                            64-vertex window with a residue that shifts by window [0]. Changes the
+                            asynchronous result, not its speed class; the autotuner keeps it */
+  int32_t reserved[2];   /* must be zero */
+} rlc_batch_tuning;
+
+typedef struct rlc_batch_stats {
+  uint32_t graphs;          /* graphs run */
+  int32_t steps;            /* iterations of the batch (the largest per-graph iteration count) */
+  float time_ms;            /* device time of admission (labels, flags) + all steps */
+  float reserved_f;
+  uint64_t vertices;        /* of the graphs run */
+  uint64_t edges;           /* directed entries of the graphs run */
+  uint64_t traversed_edges; /* sum over graphs and iterations of the degrees of processed vertices */
+  uint64_t changed;         /* label changes over graphs and iterations */
+} rlc_batch_stats;
+
+void rlc_batch_tuning_default(rlc_batch_tuning *t);
+/* An empty arena on ctx's device: room for max_graphs graphs with vertex_capacity vertex entries
+ * (a graph of n vertices takes n+1) and edge_capacity directed entries; weighted = 0 stores no
+ * weight array (every graph must then be unweighted). */
+rlc_status rlc_batch_create(rlc_context *ctx, uint64_t vertex_capacity, uint64_t edge_capacity, uint32_t max_graphs,
+                            int32_t weighted, rlc_batch **out);
+/* Device bytes a graph takes in an arena (graph arrays and per-vertex state). */
+uint64_t rlc_batch_footprint(uint64_t n, uint64_t m, int32_t weighted, int32_t sync);
+/* Copy a host CSR (same rules as rlc_graph_upload, local ids 0..n-1) into free arena space.
+ * RLC_ERR_CAPACITY when no free range or graph id is left. */
+rlc_status rlc_batch_add(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
+                         const float *weights, uint32_t *id);
+/* Release a graph's arena space and id. */
+rlc_status rlc_batch_remove(rlc_batch *b, uint32_t id);
+/* Run graphs ids[0..count) (ids == NULL: every graph of the arena) to convergence with
+ * options o (repeat and timing are ignored; no_rescan must be 0; merge_parts 4, 16 or 64).
+ * ASYNC: each iteration runs as 8 sub-sweeps, window j of 64 consecutive vertices of a graph in
+ * sub-sweep j % 8, so that later windows read the labels earlier ones wrote (options.sweeps = 1:
+ * a single sweep). t == NULL: the batch's tuning (defaults, or the last rlc_batch_tune result).
+ * stats may be NULL. */
+rlc_status rlc_batch_run(rlc_batch *b, const uint32_t *ids, uint32_t count, const rlc_options *o,
+                         const rlc_batch_tuning *t, rlc_batch_stats *stats);
+/* Labels (n entries, local ids) and iteration count of graph id after its last run. */
+rlc_status rlc_batch_result(rlc_batch *b, uint32_t id, uint32_t *labels, int32_t *iterations);
+/* Autotuner: times the first steps of a run of ids[0..count) (NULL: all) under a set of
+ * candidate tunings (grid size, items per pop, and in ASYNC the degree classes) and keeps the
+ * fastest as the batch's tuning; best (may be NULL) receives it. Leaves the graphs' labels in
+ * an unspecified state. */
+rlc_status rlc_batch_tune(rlc_batch *b, const uint32_t *ids, uint32_t count, const rlc_options *o,
+                          rlc_batch_tuning *best);
+void rlc_batch_destroy(rlc_batch *b);
+
 /* ---- synthetic inputs (host, seeded, deterministic for any thread count) ------------------- */
 typedef struct rlc_csr {
   uint64_t n, m;
@@ -191,6 +260,23 @@ rlc_status rlc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b
  * `drop`, plus shortcuts (fraction `shortcut` of the kept lattice edges) to uniform random vertices. */
 rlc_status rlc_gen_grid(uint32_t rows, uint32_t cols, double drop, double shortcut, uint64_t seed, rlc_csr *out);
 void rlc_csr_release(rlc_csr *g);
+
+/* Many graphs in one host buffer: graph i has n_i = vstart[i+1] - vstart[i] vertices, its
+ * n_i + 1 local offsets at offsets[vstart[i] + i ..] and its m_i = estart[i+1] - estart[i]
+ * targets (local ids) at edges[estart[i] ..]; weights is NULL (unit weights). */
+typedef struct rlc_csr_set {
+  uint32_t count;
+  uint64_t *vstart;  /* count+1 */
+  uint64_t *estart;  /* count+1 */
+  uint64_t *offsets; /* vstart[count] + count */
+  uint32_t *edges;   /* estart[count] */
+  float *weights;    /* NULL */
+} rlc_csr_set;
+/* count stochastic block models, graph i as rlc_gen_sbm(n[i], blocks[i], degree[i], intra[i],
+ * seeds[i]) (identical output), generated in parallel over graphs. */
+rlc_status rlc_gen_sbm_set(uint32_t count, const uint64_t *n, const uint32_t *blocks, const double *degree,
+                           const double *intra, const uint64_t *seeds, rlc_csr_set *out);
+void rlc_csr_set_release(rlc_csr_set *s);
 
 #ifdef __cplusplus
 }

@@ -15,20 +15,11 @@
 #include <string>
 #include <vector>
 
+#include "context.hpp"
 #include "host.hpp"
 #include "lpa.hpp"
 
 using namespace rlc;
-
-#define HIP_TRY(expr)                                                                              \
-  do {                                                                                             \
-    hipError_t e_ = (expr);                                                                        \
-    if (e_ != hipSuccess) return fail(RLC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-struct rlc_context {
-  int device;
-};
 
 namespace {
 
@@ -54,18 +45,6 @@ struct ExecKey {
            std::equal(group_degree, group_degree + 4, o.group_degree);
   }
 };
-
-template <class T>
-rlc_status dev_alloc(T **p, uint64_t count) {
-  size_t bytes = std::max<size_t>((size_t)count * sizeof(T), 256);
-  hipError_t e = hipMalloc((void **)p, bytes);
-  if (e != hipSuccess) {
-    *p = nullptr;
-    return fail(e == hipErrorOutOfMemory ? RLC_ERR_OUT_OF_MEMORY : RLC_ERR_DEVICE,
-                std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
-  }
-  return RLC_OK;
-}
 
 }  // namespace
 
@@ -283,7 +262,9 @@ Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
   return sc;
 }
 
-rlc_status check_options(const rlc_options *o) {
+}  // namespace
+
+rlc_status rlc::check_options(const rlc_options *o) {
   if (!o) return fail(RLC_ERR_INVALID_ARGUMENT, "options is null");
   if (o->repeat < 1) return fail(RLC_ERR_INVALID_ARGUMENT, "options.repeat must be >= 1");
   if (o->max_iterations < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.max_iterations must be >= 0");
@@ -313,14 +294,6 @@ rlc_status check_options(const rlc_options *o) {
     if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.reserved must be zero");
   return RLC_OK;
 }
-
-rlc_status check_launch() {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
-  return RLC_OK;
-}
-
-}  // namespace
 
 extern "C" {
 

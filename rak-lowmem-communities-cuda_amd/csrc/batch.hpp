@@ -1,0 +1,94 @@
+// Internal interface of the batch engine: many graphs resident in one CSR arena, advanced
+// together one label-propagation iteration per step by a persistent kernel with a work queue.
+//
+// Arena: global vertex ids (uint32) and absolute edge positions (uint64). A graph with n
+// vertices holds n+1 consecutive vertex entries [vb, vb+n]: off[vb..vb+n] are its edge
+// positions, and entry vb+n (never processed) only carries the end offset, so that graphs can
+// sit anywhere in the arena. Its targets are stored as global ids (local id + vb) and its labels
+// are global ids too; equality and order of labels within one graph are those of the local ids,
+// so every per-graph decision equals that of a run on the graph alone.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lpa.hpp"
+
+namespace rlc {
+
+enum : uint32_t { kSlotFree = 0, kSlotIdle = 1, kSlotRunning = 2, kSlotDone = 3 };
+
+// One graph of the arena (device table, indexed by graph id).
+struct BatchSlot {
+  uint32_t vb;     // first vertex
+  uint32_t n;      // vertices
+  uint32_t iter;   // iterations completed
+  uint32_t state;  // kSlot*
+  unsigned long long changed;       // moves of the current iteration
+  unsigned long long last_changed;  // moves of the last completed iteration
+};
+
+// Work queue of one step. With `sweeps` = 8 a graph is cut into rows of 512 consecutive
+// vertices, each row into 8 sets of 64 vertices (interleave: 64-vertex windows, every 8th
+// vertex, or a mix) that run in 8 consecutive launches (sub-sweeps), so that later sub-sweeps
+// read the labels earlier ones wrote (the asynchronous mode's Gauss-Seidel order at set
+// granularity); with `sweeps` = 1 a row is a 64-vertex window. Items of a sub-sweep are the
+// rows of the running graphs, numbered through prefix[] over the active list.
+constexpr int kMaxSweeps = 8;
+struct BatchQueue {
+  unsigned long long head[kMaxSweeps];  // next item to take, per sub-sweep
+  unsigned long long nitems;            // items (rows) of each sub-sweep of this step
+  uint32_t nact;                        // running graphs this step
+  uint32_t pad;
+};
+
+// Execution parameters (do not change synchronous-mode results).
+struct BatchTune {
+  uint32_t blocks;        // persistent grid (workgroups of 256 lanes)
+  uint32_t grab;          // items per queue pop
+  uint32_t group_degree;  // degree >= group_degree: group_lanes lanes per vertex (0xFFFFFFFF: never)
+  uint32_t group_lanes;   // 4 or 16
+  uint32_t wave_degree;   // degree >= wave_degree: 64 lanes per vertex (0xFFFFFFFF: never)
+};
+
+struct BatchArgs {
+  DevGraph g;         // the arena (n, m: capacities)
+  DevState s;         // lab, aff, moves (synchronous mode), ctr (totals, done, nmoves)
+  BatchSlot *slots;
+  uint32_t nslots;
+  uint32_t *active;   // running graph ids of this step
+  uint64_t *prefix;   // item prefix over active (nslots + 1)
+  BatchQueue *q;
+  int pickstep;
+  double tolerance;
+  int max_iterations;
+  unsigned long long *bstats;  // run statistics per move-kernel workgroup: (A, E, C, X)
+  uint32_t nbstats;             // workgroups with statistics slots (>= the move grid)
+  uint32_t sweeps;      // sub-sweeps per step: 1 or kMaxSweeps
+  uint32_t interleave;  // vertex sets of the sub-sweeps: 0 windows, 1 every 8th vertex, 2 mixed
+};
+
+struct BatchParams {
+  bool sync;
+  bool mg;
+};
+
+// New graphs: ids[k] (k < count) get state = running, iter = 0, labels = their global ids (or
+// init[vb..] + vb when init != null: local initial labels at the graph's arena positions) and
+// flags = 1. vfirst[k] is the prefix of their vertex counts (count + 1 entries). reset = zero
+// the run totals. Clears ctr->done.
+void launch_batch_admit(const BatchArgs &a, const uint32_t *ids, const uint64_t *vfirst, uint32_t count,
+                        uint64_t total_vertices, const uint32_t *init, bool reset, hipStream_t st);
+// A graph just copied into the arena with local values: off[0..n] += eb, edg[0..m) += vb;
+// wei_ones != null: wei_ones[0..m) = 1.
+void launch_batch_rebase(uint64_t *off, uint64_t n, uint64_t eb, uint32_t *edg, uint64_t m, uint32_t vb,
+                         float *wei_ones, hipStream_t st);
+// slots[ids[k]].state = state for k < count.
+void launch_batch_set_state(BatchSlot *slots, const uint32_t *ids, uint32_t count, uint32_t state, hipStream_t st);
+// The active list and item prefix of the next step; done when no graph is running.
+void launch_batch_plan(const BatchArgs &a, hipStream_t st);
+// One step: moves of every running graph, the apply pass (synchronous mode), per-graph
+// convergence test, then the plan of the next step.
+void launch_batch_step(const BatchArgs &a, const BatchTune &t, const BatchParams &p, hipStream_t st);
+
+}  // namespace rlc

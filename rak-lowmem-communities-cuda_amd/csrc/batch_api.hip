@@ -1,0 +1,555 @@
+// C ABI of the batch engine: a device CSR arena holding many graphs, runs over any subset of
+// them with one persistent launch per step (captured as a hipGraph), and the autotuner of the
+// launch parameters. No counterpart in the reference, which runs one graph per call
+// (rakLowmemInvokeCuda, inc/rakLowmemCuda.hxx:538-607); the per-graph results are those of
+// rlc_run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "arena.hpp"
+#include "batch.hpp"
+#include "context.hpp"
+#include "host.hpp"
+
+using namespace rlc;
+
+namespace {
+
+constexpr uint32_t kNever = 0xFFFFFFFFu;
+
+struct BatchKey {
+  bool valid = false;
+  int sync = 0, mg = 0, pickstep = 0, max_iterations = 0, sweeps = 0, interleave = 0;
+  double tolerance = 0;
+  BatchTune t{};
+  bool operator==(const BatchKey &o) const {
+    return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
+           max_iterations == o.max_iterations && sweeps == o.sweeps && interleave == o.interleave &&
+           tolerance == o.tolerance &&
+           t.blocks == o.t.blocks &&
+           t.grab == o.t.grab && t.group_degree == o.t.group_degree && t.group_lanes == o.t.group_lanes &&
+           t.wave_degree == o.t.wave_degree;
+  }
+};
+
+struct HostSlot {
+  bool used = false;
+  uint64_t vb = 0, n = 0, eb = 0, m = 0;
+};
+
+struct BusyFlag {
+  std::atomic<int> &b;
+  bool ok;
+  explicit BusyFlag(std::atomic<int> &x) : b(x) {
+    int z = 0;
+    ok = b.compare_exchange_strong(z, 1);
+  }
+  ~BusyFlag() {
+    if (ok) b.store(0);
+  }
+};
+
+}  // namespace
+
+struct rlc_batch {
+  rlc_context *ctx = nullptr;
+  uint64_t vcap = 0, ecap = 0;
+  uint32_t maxg = 0;
+  bool weighted = false;
+  int cus = 256;
+  // arena
+  uint64_t *d_off = nullptr;
+  uint32_t *d_edg = nullptr;
+  float *d_wei = nullptr;
+  uint32_t *d_lab = nullptr;
+  uint8_t *d_aff = nullptr;
+  uint64_t *d_moves = nullptr;
+  // tables
+  BatchSlot *d_slots = nullptr;
+  uint32_t *d_active = nullptr;
+  uint64_t *d_prefix = nullptr;
+  BatchQueue *d_q = nullptr;
+  DevCounters *d_ctr = nullptr;
+  uint32_t *d_ids = nullptr;
+  uint64_t *d_vfirst = nullptr;
+  unsigned long long *d_bstats = nullptr;
+  uint32_t nbstats = 0;
+  // host bookkeeping
+  std::vector<HostSlot> slots;
+  std::vector<uint32_t> free_ids;  // stack, smallest id on top
+  RangeAlloc valloc, ealloc;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t hg = nullptr;
+  hipGraphExec_t hx = nullptr;
+  BatchKey key;
+  rlc_batch_tuning tuning{};
+  std::atomic<int> busy{0};
+
+  BatchArgs args(const rlc_options &o, const rlc_batch_tuning &t) const {
+    BatchArgs a;
+    a.g = DevGraph{d_off, d_edg, weighted ? d_wei : nullptr, vcap, ecap};
+    a.s = DevState{d_lab, nullptr, d_aff, d_moves, d_ctr};
+    a.slots = d_slots;
+    a.nslots = maxg;
+    a.active = d_active;
+    a.prefix = d_prefix;
+    a.q = d_q;
+    a.pickstep = o.pickstep;
+    a.tolerance = o.tolerance;
+    a.max_iterations = o.max_iterations;
+    a.bstats = d_bstats;
+    a.nbstats = nbstats;
+    // asynchronous mode: kMaxSweeps sub-sweeps per step (options.sweeps = 1: a single sweep);
+    // the synchronous mode reads a snapshot, so one sweep
+    a.sweeps = (o.mode == RLC_MODE_SYNC || o.sweeps == 1) ? 1u : (uint32_t)kMaxSweeps;
+    a.interleave = (uint32_t)t.interleave;
+    return a;
+  }
+};
+
+namespace {
+
+void free_batch(rlc_batch *b) {
+  if (!b) return;
+  if (b->ctx) (void)hipSetDevice(b->ctx->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->hx) (void)hipGraphExecDestroy(b->hx);
+  if (b->hg) (void)hipGraphDestroy(b->hg);
+  void *ptrs[] = {b->d_off, b->d_edg, b->d_wei, b->d_lab, b->d_aff, b->d_moves, b->d_slots,
+                  b->d_active, b->d_prefix, b->d_q, b->d_ctr, b->d_ids, b->d_vfirst, b->d_bstats};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto &e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+rlc_status check_tuning(const rlc_batch_tuning *t) {
+  if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.blocks_per_cu must be in 1..16");
+  if (t->grab < 1 || t->grab > 256) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.grab must be in 1..256");
+  if (t->group_degree < 0 || t->wave_degree < 0)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "tuning degrees must be >= 0");
+  if (t->group_lanes != 4 && t->group_lanes != 16)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.group_lanes must be 4 or 16");
+  if (t->interleave < 0 || t->interleave > 2) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.interleave must be 0, 1 or 2");
+  for (int r : t->reserved)
+    if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.reserved must be zero");
+  return RLC_OK;
+}
+
+rlc_status check_batch_options(const rlc_options *o) {
+  rlc_status st = check_options(o);
+  if (st != RLC_OK) return st;
+  if (o->no_rescan) return fail(RLC_ERR_INVALID_ARGUMENT, "options.no_rescan is not supported by batches");
+  if (o->mode == RLC_MODE_SYNC && o->merge_degree > 0 && o->merge_parts == 256)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts = 256 is not supported by batches (4, 16 or 64)");
+  return RLC_OK;
+}
+
+// Degree classes and grid of a run. SYNC: the classes are part of the result (merge tree of
+// options.merge_parts parts for degree >= merge_degree), exactly as rlc_run builds them.
+BatchTune make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t) {
+  BatchTune bt;
+  bt.blocks = (uint32_t)(b->cus * t.blocks_per_cu);
+  bt.grab = (uint32_t)t.grab;
+  if (o.mode == RLC_MODE_SYNC) {
+    bt.group_degree = kNever;
+    bt.wave_degree = kNever;
+    bt.group_lanes = 4;
+    if (o.merge_degree > 0) {
+      if (o.merge_parts == 64) {
+        bt.wave_degree = (uint32_t)o.merge_degree;
+      } else {
+        bt.group_degree = (uint32_t)o.merge_degree;
+        bt.group_lanes = (uint32_t)o.merge_parts;
+      }
+    }
+  } else {
+    bt.group_degree = t.group_degree > 0 ? (uint32_t)t.group_degree : kNever;
+    bt.wave_degree = t.wave_degree > 0 ? (uint32_t)t.wave_degree : kNever;
+    bt.group_lanes = (uint32_t)t.group_lanes;
+  }
+  return bt;
+}
+
+// Selected graph ids (all used ones when ids is null), validated, and their vertex prefix.
+rlc_status select_ids(const rlc_batch *b, const uint32_t *ids, uint32_t count, std::vector<uint32_t> &sel,
+                      std::vector<uint64_t> &vfirst) {
+  sel.clear();
+  if (ids) {
+    std::vector<uint8_t> seen(b->maxg, 0);
+    for (uint32_t k = 0; k < count; ++k) {
+      const uint32_t id = ids[k];
+      if (id >= b->maxg || !b->slots[id].used)
+        return fail(RLC_ERR_INVALID_ARGUMENT, "batch: graph id " + std::to_string(id) + " is not in the arena");
+      if (seen[id]) return fail(RLC_ERR_INVALID_ARGUMENT, "batch: graph id " + std::to_string(id) + " repeated");
+      seen[id] = 1;
+      sel.push_back(id);
+    }
+  } else {
+    for (uint32_t id = 0; id < b->maxg; ++id)
+      if (b->slots[id].used) sel.push_back(id);
+  }
+  vfirst.assign(sel.size() + 1, 0);
+  for (size_t k = 0; k < sel.size(); ++k) vfirst[k + 1] = vfirst[k] + b->slots[sel[k]].n;
+  return RLC_OK;
+}
+
+rlc_status upload_selection(rlc_batch *b, const std::vector<uint32_t> &sel, const std::vector<uint64_t> &vfirst) {
+  if (sel.empty()) return RLC_OK;
+  HIP_TRY(hipMemcpyAsync(b->d_ids, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipMemcpyAsync(b->d_vfirst, vfirst.data(), vfirst.size() * 8, hipMemcpyHostToDevice, b->stream));
+  return RLC_OK;
+}
+
+rlc_status ensure_moves(rlc_batch *b, const rlc_options &o) {
+  if (o.mode != RLC_MODE_SYNC || b->d_moves) return RLC_OK;
+  rlc_status st = dev_alloc(&b->d_moves, b->vcap);
+  if (st != RLC_OK) return st;
+  b->key.valid = false;  // captured launches embed the state pointers
+  return RLC_OK;
+}
+
+// Admission of the selection + plan of the first step, then `steps` steps enqueued directly.
+rlc_status enqueue_probe(rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t, uint32_t count,
+                         uint64_t total, int steps) {
+  const BatchTune bt = make_tune(b, o, t);
+  const BatchArgs a = b->args(o, t);
+  const BatchParams p{o.mode == RLC_MODE_SYNC, o.slots == RLC_SKETCH_MG8};
+  launch_batch_admit(a, b->d_ids, b->d_vfirst, count, total, nullptr, true, b->stream);
+  launch_batch_plan(a, b->stream);
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  for (int l = 0; l < steps; ++l) launch_batch_step(a, bt, p, b->stream);
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  return check_launch();
+}
+
+}  // namespace
+
+extern "C" {
+
+void rlc_batch_tuning_default(rlc_batch_tuning *t) {
+  if (!t) return;
+  std::memset(t, 0, sizeof(*t));
+  t->blocks_per_cu = 4;
+  t->grab = 4;
+  t->group_degree = 32;
+  t->group_lanes = 4;
+  t->wave_degree = 256;
+  t->interleave = 0;
+}
+
+uint64_t rlc_batch_footprint(uint64_t n, uint64_t m, int32_t weighted, int32_t sync) {
+  // offsets (8), labels (4), flags (1), change list (8, synchronous mode) per vertex entry;
+  // target (4) and weight (4) per directed entry
+  return (n + 1) * (8 + 4 + 1 + (sync ? 8 : 0)) + m * (4 + (weighted ? 4 : 0));
+}
+
+rlc_status rlc_batch_create(rlc_context *ctx, uint64_t vertex_capacity, uint64_t edge_capacity, uint32_t max_graphs,
+                            int32_t weighted, rlc_batch **out) {
+  clear_error();
+  if (!ctx || !out) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_create: null argument");
+  *out = nullptr;
+  if (max_graphs == 0) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_create: max_graphs must be > 0");
+  if (vertex_capacity >= 0xFFFFFFFFull)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_create: vertex_capacity must be below 2^32-1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  rlc_batch *b = new rlc_batch();
+  b->ctx = ctx;
+  b->vcap = vertex_capacity;
+  b->ecap = edge_capacity;
+  b->maxg = max_graphs;
+  b->weighted = weighted != 0;
+  rlc_batch_tuning_default(&b->tuning);
+  auto bail = [&](rlc_status s) {
+    free_batch(b);
+    return s;
+  };
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
+    b->cus = cus;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "hipStreamCreate failed"));
+  for (auto &e : b->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
+  rlc_status st;
+  if ((st = dev_alloc(&b->d_off, vertex_capacity + 1)) != RLC_OK ||
+      (st = dev_alloc(&b->d_edg, edge_capacity)) != RLC_OK ||
+      (b->weighted && (st = dev_alloc(&b->d_wei, edge_capacity)) != RLC_OK) ||
+      (st = dev_alloc(&b->d_lab, vertex_capacity)) != RLC_OK || (st = dev_alloc(&b->d_aff, vertex_capacity)) != RLC_OK ||
+      (st = dev_alloc(&b->d_slots, max_graphs)) != RLC_OK || (st = dev_alloc(&b->d_active, max_graphs)) != RLC_OK ||
+      (st = dev_alloc(&b->d_prefix, (uint64_t)max_graphs + 1)) != RLC_OK || (st = dev_alloc(&b->d_q, 1)) != RLC_OK ||
+      (st = dev_alloc(&b->d_ctr, 1)) != RLC_OK || (st = dev_alloc(&b->d_ids, max_graphs)) != RLC_OK ||
+      (st = dev_alloc(&b->d_vfirst, (uint64_t)max_graphs + 1)) != RLC_OK)
+    return bail(st);
+  b->nbstats = (uint32_t)b->cus * 16;  // the largest move grid (blocks_per_cu <= 16)
+  if ((st = dev_alloc(&b->d_bstats, 4ull * b->nbstats)) != RLC_OK) return bail(st);
+  // every slot free, counters zero; flags zero so that an unused arena entry is never active
+  if (hipMemsetAsync(b->d_slots, 0, (size_t)max_graphs * sizeof(BatchSlot), b->stream) != hipSuccess ||
+      hipMemsetAsync(b->d_ctr, 0, sizeof(DevCounters), b->stream) != hipSuccess ||
+      hipMemsetAsync(b->d_q, 0, sizeof(BatchQueue), b->stream) != hipSuccess ||
+      (vertex_capacity && hipMemsetAsync(b->d_aff, 0, vertex_capacity, b->stream) != hipSuccess) ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "batch initialisation failed"));
+  b->slots.assign(max_graphs, HostSlot{});
+  b->free_ids.resize(max_graphs);
+  for (uint32_t i = 0; i < max_graphs; ++i) b->free_ids[i] = max_graphs - 1 - i;
+  b->valloc.reset(vertex_capacity);
+  b->ealloc.reset(edge_capacity);
+  *out = b;
+  return RLC_OK;
+}
+
+rlc_status rlc_batch_add(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
+                         const float *weights, uint32_t *id) {
+  clear_error();
+  if (!b || !offsets || !id) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_add: null argument");
+  BusyFlag guard(b->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_batch_add: batch is in use by another call");
+  uint64_t md = 0;
+  rlc_status st = validate_csr(n, m, offsets, edges, weights, &md);
+  if (st != RLC_OK) return st;
+  if (weights && !b->weighted) {
+    for (uint64_t i = 0; i < m; ++i)
+      if (weights[i] != 1.0f) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_add: the batch stores no weights (weighted = 0)");
+  }
+  if (b->free_ids.empty()) return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free graph id");
+  uint64_t vb = 0, eb = 0;
+  if (!b->valloc.alloc(n + 1, vb)) return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free vertex range");
+  if (!b->ealloc.alloc(m, eb)) {
+    b->valloc.release(vb, n + 1);
+    return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free edge range");
+  }
+  const uint32_t g = b->free_ids.back();
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  auto undo = [&](rlc_status s) {
+    b->valloc.release(vb, n + 1);
+    b->ealloc.release(eb, m);
+    return s;
+  };
+  const BatchSlot slot{(uint32_t)vb, (uint32_t)n, 0u, (uint32_t)kSlotIdle, 0ull, 0ull};
+  if (hipMemcpyAsync(b->d_off + vb, offsets, (n + 1) * 8, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+      (m && hipMemcpyAsync(b->d_edg + eb, edges, m * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess) ||
+      (m && b->weighted && weights &&
+       hipMemcpyAsync(b->d_wei + eb, weights, m * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess) ||
+      hipMemcpyAsync(b->d_slots + g, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+    return undo(fail(RLC_ERR_DEVICE, "rlc_batch_add: copy failed"));
+  launch_batch_rebase(b->d_off + vb, n, eb, b->d_edg + eb, m, (uint32_t)vb,
+                      b->weighted && !weights ? b->d_wei + eb : nullptr, b->stream);
+  if ((st = check_launch()) != RLC_OK) return undo(st);
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return undo(fail(RLC_ERR_DEVICE, "rlc_batch_add: synchronise failed"));
+  b->free_ids.pop_back();
+  b->slots[g] = HostSlot{true, vb, n, eb, m};
+  *id = g;
+  return RLC_OK;
+}
+
+rlc_status rlc_batch_remove(rlc_batch *b, uint32_t id) {
+  clear_error();
+  if (!b) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_remove: null batch");
+  BusyFlag guard(b->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_batch_remove: batch is in use by another call");
+  if (id >= b->maxg || !b->slots[id].used) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_remove: unknown graph id");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  const BatchSlot slot{0u, 0u, 0u, (uint32_t)kSlotFree, 0ull, 0ull};
+  HIP_TRY(hipMemcpyAsync(b->d_slots + id, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  const HostSlot &h = b->slots[id];
+  b->valloc.release(h.vb, h.n + 1);
+  b->ealloc.release(h.eb, h.m);
+  b->slots[id] = HostSlot{};
+  b->free_ids.push_back(id);
+  std::sort(b->free_ids.begin(), b->free_ids.end(), std::greater<uint32_t>());
+  return RLC_OK;
+}
+
+rlc_status rlc_batch_run(rlc_batch *b, const uint32_t *ids, uint32_t count, const rlc_options *o,
+                         const rlc_batch_tuning *t, rlc_batch_stats *stats) {
+  clear_error();
+  if (!b) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_run: null batch");
+  rlc_status st = check_batch_options(o);
+  if (st != RLC_OK) return st;
+  if (t && (st = check_tuning(t)) != RLC_OK) return st;
+  BusyFlag guard(b->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_batch_run: batch is in use by another call");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  std::vector<uint32_t> sel;
+  std::vector<uint64_t> vfirst;
+  if ((st = select_ids(b, ids, count, sel, vfirst)) != RLC_OK) return st;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (sel.empty()) return RLC_OK;
+  if ((st = ensure_moves(b, *o)) != RLC_OK) return st;
+  if ((st = upload_selection(b, sel, vfirst)) != RLC_OK) return st;
+  const rlc_batch_tuning &tu = t ? *t : b->tuning;
+  const BatchTune bt = make_tune(b, *o, tu);
+  const BatchArgs a = b->args(*o, tu);
+  const BatchParams p{o->mode == RLC_MODE_SYNC, o->slots == RLC_SKETCH_MG8};
+  BatchKey k;
+  k.valid = true;
+  k.sync = p.sync;
+  k.mg = p.mg;
+  k.pickstep = o->pickstep;
+  k.max_iterations = o->max_iterations;
+  k.sweeps = (int)a.sweeps;
+  k.interleave = (int)a.interleave;
+  k.tolerance = o->tolerance;
+  k.t = bt;
+  if (o->max_iterations > 0 && !(k == b->key)) {
+    if (b->hx) { (void)hipGraphExecDestroy(b->hx); b->hx = nullptr; }
+    if (b->hg) { (void)hipGraphDestroy(b->hg); b->hg = nullptr; }
+    b->key.valid = false;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    for (int l = 0; l < o->max_iterations; ++l) launch_batch_step(a, bt, p, b->stream);
+    hipGraph_t hg = nullptr;
+    hipError_t ce = hipStreamEndCapture(b->stream, &hg);
+    if (ce != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    b->hg = hg;
+    HIP_TRY(hipGraphInstantiate(&b->hx, b->hg, nullptr, nullptr, 0));
+    b->key = k;
+  }
+  HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+  launch_batch_admit(a, b->d_ids, b->d_vfirst, (uint32_t)sel.size(), vfirst.back(), nullptr, true, b->stream);
+  if (o->max_iterations > 0) {
+    launch_batch_plan(a, b->stream);
+    if ((st = check_launch()) != RLC_OK) return st;
+    HIP_TRY(hipGraphLaunch(b->hx, b->stream));
+  } else {
+    // no iteration: the labels stay the vertex ids (as rlc_run with max_iterations = 0)
+    launch_batch_set_state(b->d_slots, b->d_ids, (uint32_t)sel.size(), kSlotDone, b->stream);
+  }
+  HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+  HIP_TRY(hipEventSynchronize(b->ev[1]));
+  if ((st = check_launch()) != RLC_OK) return st;
+  if (stats) {
+    DevCounters hc;
+    HIP_TRY(hipMemcpy(&hc, b->d_ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    stats->graphs = (uint32_t)sel.size();
+    stats->steps = (int32_t)hc.iter;
+    stats->time_ms = ms;
+    stats->vertices = vfirst.back();
+    uint64_t e = 0;
+    for (uint32_t id : sel) e += b->slots[id].m;
+    stats->edges = e;
+    std::vector<unsigned long long> bs(4ull * b->nbstats);
+    HIP_TRY(hipMemcpy(bs.data(), b->d_bstats, bs.size() * 8, hipMemcpyDeviceToHost));
+    uint64_t E = 0, C = 0;
+    for (uint32_t i = 0; i < b->nbstats; ++i) {
+      E += bs[4ull * i + 1];
+      C += bs[4ull * i + 2];
+    }
+    stats->traversed_edges = E;
+    stats->changed = C;
+  }
+  return RLC_OK;
+}
+
+rlc_status rlc_batch_result(rlc_batch *b, uint32_t id, uint32_t *labels, int32_t *iterations) {
+  clear_error();
+  if (!b) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_result: null batch");
+  BusyFlag guard(b->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_batch_result: batch is in use by another call");
+  if (id >= b->maxg || !b->slots[id].used) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_result: unknown graph id");
+  const HostSlot &h = b->slots[id];
+  if (!labels && h.n) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_result: labels is null");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  BatchSlot s;
+  HIP_TRY(hipMemcpyAsync(&s, b->d_slots + id, sizeof(s), hipMemcpyDeviceToHost, b->stream));
+  if (h.n) HIP_TRY(hipMemcpyAsync(labels, b->d_lab + h.vb, h.n * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if (s.state == kSlotIdle) return fail(RLC_ERR_NOT_READY, "rlc_batch_result: graph has not been run");
+  const uint32_t vb = (uint32_t)h.vb;
+  for (uint64_t i = 0; i < h.n; ++i) labels[i] -= vb;
+  if (iterations) *iterations = (int32_t)s.iter;
+  return RLC_OK;
+}
+
+rlc_status rlc_batch_tune(rlc_batch *b, const uint32_t *ids, uint32_t count, const rlc_options *o,
+                          rlc_batch_tuning *best) {
+  clear_error();
+  if (!b) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_tune: null batch");
+  rlc_status st = check_batch_options(o);
+  if (st != RLC_OK) return st;
+  BusyFlag guard(b->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_batch_tune: batch is in use by another call");
+  HIP_TRY(hipSetDevice(b->ctx->device));
+  std::vector<uint32_t> sel;
+  std::vector<uint64_t> vfirst;
+  if ((st = select_ids(b, ids, count, sel, vfirst)) != RLC_OK) return st;
+  rlc_batch_tuning cur = b->tuning;
+  if (sel.empty() || o->max_iterations <= 0) {
+    if (best) *best = cur;
+    return RLC_OK;
+  }
+  if ((st = ensure_moves(b, *o)) != RLC_OK) return st;
+  if ((st = upload_selection(b, sel, vfirst)) != RLC_OK) return st;
+  // probe: the first steps of a run (the iterations with most active vertices), best of 2
+  const int steps = std::min(o->max_iterations, 3);
+  auto probe = [&](const rlc_batch_tuning &t, float &ms) -> rlc_status {
+    ms = 1e30f;
+    for (int r = 0; r < 2; ++r) {
+      rlc_status s2 = enqueue_probe(b, *o, t, (uint32_t)sel.size(), vfirst.back(), steps);
+      if (s2 != RLC_OK) return s2;
+      HIP_TRY(hipEventSynchronize(b->ev[1]));
+      float x = 0;
+      HIP_TRY(hipEventElapsedTime(&x, b->ev[0], b->ev[1]));
+      ms = std::min(ms, x);
+    }
+    return RLC_OK;
+  };
+  float best_ms = 0;
+  if ((st = probe(cur, best_ms)) != RLC_OK) return st;
+  auto try_one = [&](rlc_batch_tuning t) -> rlc_status {
+    float ms = 0;
+    rlc_status s2 = probe(t, ms);
+    if (s2 != RLC_OK) return s2;
+    if (ms < best_ms) {
+      best_ms = ms;
+      cur = t;
+    }
+    return RLC_OK;
+  };
+  // coordinate descent: items per pop, then grid size, then (asynchronous mode) degree classes
+  for (int g : {1, 2, 8, 16, 64}) {
+    rlc_batch_tuning t = cur;
+    t.grab = g;
+    if (t.grab != cur.grab && (st = try_one(t)) != RLC_OK) return st;
+  }
+  for (int bpc : {1, 2, 8}) {
+    rlc_batch_tuning t = cur;
+    t.blocks_per_cu = bpc;
+    if (t.blocks_per_cu != cur.blocks_per_cu && (st = try_one(t)) != RLC_OK) return st;
+  }
+  if (o->mode == RLC_MODE_ASYNC) {
+    const int classes[][3] = {{0, 4, 0},     {16, 4, 128},  {32, 4, 0},     {32, 4, 512},
+                              {64, 16, 256}, {32, 16, 512}, {128, 16, 1024}, {64, 4, 2048}};
+    for (auto &c : classes) {
+      rlc_batch_tuning t = cur;
+      t.group_degree = c[0];
+      t.group_lanes = c[1];
+      t.wave_degree = c[2];
+      if ((st = try_one(t)) != RLC_OK) return st;
+    }
+  }
+  // the probes leave graphs mid-run: back to "not run"
+  launch_batch_set_state(b->d_slots, b->d_ids, (uint32_t)sel.size(), kSlotIdle, b->stream);
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  if ((st = check_launch()) != RLC_OK) return st;
+  b->tuning = cur;
+  if (best) *best = cur;
+  return RLC_OK;
+}
+
+void rlc_batch_destroy(rlc_batch *b) { free_batch(b); }
+
+}  // extern "C"

@@ -1,0 +1,530 @@
+// HIP kernels of the batch engine (gfx950, wave64): one persistent move kernel that takes
+// 64-vertex work items of every running graph from a device-wide queue, plus the admit / plan /
+// finish steps that give each graph its own iteration count and convergence test.
+//
+// Per vertex the semantics are those of the single-graph kernels (and of the reference's move
+// kernels, inc/rakLowmemCuda.hxx:138-161, :269-317, :362-419): scan (label, weight) of every
+// non-self edge into a Misra-Gries (8 slots) or Boyer-Moore sketch, rescan the candidates when
+// the sketch cannot certify its choice, accept c if w > 0, c != d and, on pick-less iterations,
+// c < d; clear the vertex's flag, and flag the neighbours of a moved vertex. A vertex is scanned
+// by one lane, by a group of 4 or 16 lanes, or by the whole wavefront, depending on its degree;
+// a group's P strided parts are merged by the fixed binary tree of the single-graph kernels, so
+// in the synchronous mode every label equals that of rlc_run on the graph alone.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "batch.hpp"
+#include "edges.hpp"
+#include "sketch.hpp"
+
+namespace rlc {
+namespace {
+
+// Run statistics: each workgroup adds its wave totals to its own four counters (no atomics:
+// only this workgroup index writes them, launch after launch).
+__device__ __forceinline__ void block_stats(unsigned long long *out, unsigned long long a, unsigned long long e,
+                                            unsigned long long c, unsigned long long x) {
+  __shared__ unsigned long long red[4][kBlock / 64];
+  a = wave_sum_u64(a);
+  e = wave_sum_u64(e);
+  c = wave_sum_u64(c);
+  x = wave_sum_u64(x);
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = a;
+    red[1][wave] = e;
+    red[2][wave] = c;
+    red[3][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
+    out[threadIdx.x] += t;
+  }
+}
+
+// ---- per-vertex choice -------------------------------------------------------------------------
+// One lane scans vertex u sequentially (CSR order); MG8 candidates are rescanned by gathering
+// the labels again (the labels a vertex reads do not change during a synchronous step).
+template <bool MG, bool UNIT>
+__device__ __forceinline__ void choose_seq(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t b, uint64_t e,
+                                           uint32_t &cb, float &wb) {
+  if (MG) {
+    MG8 sk;
+    sk.clear();
+    for_edges<UNIT, 1>(g, lab, u, b, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
+    if (!certain_argmax(sk.c, sk.w, sk.dsum, e - b, cb, wb)) {
+      float acc[kSlots];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
+      for_edges<UNIT, 1>(g, lab, u, b, e, [&](uint32_t c, float w) {
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) acc[j] = sk.c[j] == c ? acc[j] + w : acc[j];
+      });
+      argmax8(sk.c, acc, cb, wb);
+    }
+  } else {
+    uint32_t mc = kEmpty;
+    float mw = 0.0f;
+    for_edges<UNIT, 1>(g, lab, u, b, e, [&](uint32_t c, float w) { bm_step(mc, mw, c, w); });
+    cb = mc;
+    wb = mw;
+  }
+}
+
+// W aligned lanes scan vertex u: part gl takes positions b+gl, b+gl+W, ..; the parts' sketches
+// are merged by group_merge_tree (part p absorbs p+k), the decrement weights summed by the
+// butterfly, the candidates' exact weights summed by the same butterfly. Every lane of the group
+// returns the same (cb, wb).
+template <bool MG, bool UNIT, int W>
+__device__ __forceinline__ void choose_group(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t b, uint64_t e,
+                                             int gl, uint32_t &cb, float &wb) {
+  const int leader = lane_id() & ~(W - 1);
+  if (MG) {
+    MG8 sk;
+    sk.clear();
+    for_edges<UNIT, W>(g, lab, u, b + gl, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
+    group_merge_tree<W>(sk);
+    uint32_t cand[kSlots];
+    float cw[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      cand[j] = __shfl(sk.c[j], leader);
+      cw[j] = __shfl(sk.w[j], leader);
+    }
+    float dtot = sk.dsum;
+#pragma unroll
+    for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
+    if (!certain_argmax(cand, cw, dtot, e - b, cb, wb)) {
+      float acc[kSlots];
+#pragma unroll
+      for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
+      for_edges<UNIT, W>(g, lab, u, b + gl, e, [&](uint32_t c, float w) {
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == c ? acc[j] + w : acc[j];
+      });
+      group_sum8<W>(acc);
+      argmax8(cand, acc, cb, wb);
+    }
+  } else {
+    uint32_t mc = kEmpty;
+    float mw = 0.0f;
+    for_edges<UNIT, W>(g, lab, u, b + gl, e, [&](uint32_t c, float w) { bm_step(mc, mw, c, w); });
+    group_bm_tree<W>(mc, mw);
+    cb = __shfl(mc, leader);
+    wb = __shfl(mw, leader);
+  }
+}
+
+// Per-wavefront state of the move kernel: counters of the run statistics, and (synchronous
+// mode) a buffer of the wave's moves in LDS, appended to the change list kMoveBuf at a time so
+// that the device-wide counter sees one atomic per flush instead of one per item.
+constexpr uint32_t kMoveBuf = 256;
+struct WaveCtx {
+  unsigned long long a, e, c, x;  // active vertices, their edges, moves, edges of moved vertices
+  uint64_t *buf;                  // LDS: kMoveBuf entries (u << 32) | label
+  uint32_t nbuf;                  // entries held (wave-uniform)
+};
+
+__device__ __forceinline__ void flush_moves(const DevState &s, WaveCtx &w) {
+  if (w.nbuf == 0) return;
+  wave_lds_sync();
+  unsigned long long base = 0;
+  if (lane_id() == 0) base = atomicAdd(&s.ctr->nmoves, (unsigned long long)w.nbuf);
+  base = __shfl(base, 0);
+  for (uint32_t i = (uint32_t)lane_id(); i < w.nbuf; i += 64) s.moves[base + i] = w.buf[i];
+  w.nbuf = 0;
+  wave_lds_sync();
+}
+
+// Synchronous mode: (u, c) of the lanes with `lead` set, into the wave's buffer.
+__device__ __forceinline__ void append_moves(const DevState &s, WaveCtx &w, bool lead, uint32_t u, uint32_t c) {
+  const unsigned long long mask = __ballot(lead);
+  if (mask == 0ull) return;
+  if (w.nbuf + 64 > kMoveBuf) flush_moves(s, w);
+  if (lead) w.buf[w.nbuf + lanes_below(mask)] = ((uint64_t)u << 32) | c;
+  w.nbuf += (uint32_t)__popcll(mask);
+}
+
+// Vertices of the group class are taken 64/W at a time: group gi of the wave takes the gi-th
+// set bit of `mask` (a lane of the item), the processed bits are then cleared.
+template <bool SYNC, bool MG, bool UNIT, int W>
+__device__ __forceinline__ uint32_t run_groups(const BatchArgs &a, unsigned long long mask, uint32_t lu, uint64_t b,
+                                               uint64_t e, uint32_t d, bool pickless, WaveCtx &ws) {
+  const DevGraph &g = a.g;
+  const DevState &s = a.s;
+  const int lane = lane_id();
+  const int gi = lane / W, gl = lane % W;
+  uint32_t moves = 0;
+  while (mask != 0ull) {
+    unsigned long long m = mask;
+    for (int q = 0; q < gi; ++q) m &= m - 1ull;
+    const bool has = m != 0ull;
+    const int vl = has ? __builtin_ctzll(m) : 0;
+    const uint32_t u = __shfl(lu, vl);
+    const uint64_t vb = __shfl(b, vl), ve = __shfl(e, vl);
+    const uint32_t vd = __shfl(d, vl);
+    bool moved = false;
+    uint32_t cb = 0;
+    if (has) {
+      float wb = 0.0f;
+      choose_group<MG, UNIT, W>(g, s.lab, u, vb, ve, gl, cb, wb);
+      moved = wb > 0.0f && cb != vd && !(pickless && cb > vd);
+      if (gl == 0) s.aff[u] = 0;
+      if (!SYNC && moved) {
+        if (gl == 0) s.lab[u] = cb;
+        mark_neighbours(g, s.aff, u, vb + gl, ve, W);
+      }
+    }
+    const bool lead = has && gl == 0 && moved;
+    if (SYNC) append_moves(s, ws, lead, u, cb);
+    if (lead) {
+      ++moves;
+      if (!SYNC) ws.x += ve - vb;
+    }
+#pragma unroll
+    for (int q = 0; q < 64 / W; ++q) mask &= mask - 1ull;
+  }
+  return moves;
+}
+
+// Local vertex of `lane` in row r (vertices 512 r .. 512 r + 511 of a graph) for sub-sweep
+// `sub` of 8 (or of the only sweep, sub = 0 of 1, where a row is 64 vertices):
+//   window  (interleave 0)  the 64 consecutive vertices 512 r + 64 sub + lane
+//   vertex  (interleave 1)  every 8th vertex: 512 r + sub + 8 lane
+//   mixed   (interleave 2)  8 vertices of each of the row's 8 windows, the window's residue
+//                           mod 8 shifted by the window: 512 r + 64 w + 8 k + ((sub + w) & 7),
+//                           lane = 8 w + k
+__device__ __forceinline__ uint64_t row_vertex(uint32_t interleave, uint32_t sweeps, uint64_t r, uint32_t sub,
+                                               int lane) {
+  if (sweeps == 1) return r * 64u + (uint64_t)lane;
+  const uint64_t base = r * 512u;
+  if (interleave == 1) return base + sub + 8u * (uint64_t)lane;
+  if (interleave == 2) {
+    const uint32_t w = (uint32_t)lane >> 3, k = (uint32_t)lane & 7u;
+    return base + 64u * w + 8u * k + ((sub + w) & 7u);
+  }
+  return base + 64u * sub + (uint64_t)lane;
+}
+
+// Staged edges per wavefront (LDS): 4 KB of labels (and 4 KB of weights when weighted).
+constexpr int kStageCap = 1024;
+template <bool UNIT>
+using BatchStage = StageLds<UNIT, kStageCap>;
+
+// One work item: row r of graph `id` (vb: its first vertex, n: its size), one vertex per lane.
+// GW = 0: every vertex is scanned by one lane (no group or wave class: fewer registers). Lane
+// vertices of degree <= kStageCap are scanned from the wave's LDS stage.
+// Returns the item's number of moves (wave-uniform).
+template <bool SYNC, bool MG, bool UNIT, int GW>
+__device__ __forceinline__ uint32_t process_item(const BatchArgs &a, const BatchTune &t, uint32_t vb, uint32_t n,
+                                                 uint64_t r, uint32_t sub, uint32_t iter, WaveCtx &ws,
+                                                 BatchStage<UNIT> &L) {
+  const DevGraph &g = a.g;
+  const DevState &s = a.s;
+  const int lane = lane_id();
+  const bool pickless = a.pickstep > 0 && (iter % (uint32_t)a.pickstep) == 0;
+  const uint64_t lv = row_vertex(a.interleave, a.sweeps, r, sub, lane);
+  const uint32_t u = vb + (uint32_t)(lv < n ? lv : 0);
+  const bool act = lv < n && s.aff[u] != 0;
+  uint64_t b = 0, e = 0;
+  uint32_t d = 0;
+  if (act) {
+    b = g.off[u];
+    e = g.off[(uint64_t)u + 1];
+    d = s.lab[u];
+  }
+  const uint64_t deg = e - b;
+  ws.a += act ? 1 : 0;
+  ws.e += deg;
+  const int cls = !act ? -1 : (GW == 0 ? 0 : (deg >= t.wave_degree ? 2 : (deg >= t.group_degree ? 1 : 0)));
+  uint32_t moves = 0;
+  {
+    bool moved = false;
+    uint32_t cb = 0;
+    float wb = 0.0f;
+    staged_choose<MG, UNIT, kStageCap>(g, s.lab, L, cls == 0 && deg <= (uint64_t)kStageCap, u, b, e, cb, wb);
+    if (cls == 0) {
+      if (deg > (uint64_t)kStageCap) choose_seq<MG, UNIT>(g, s.lab, u, b, e, cb, wb);
+      moved = wb > 0.0f && cb != d && !(pickless && cb > d);
+      s.aff[u] = 0;
+      if (!SYNC && moved) {
+        s.lab[u] = cb;
+        mark_neighbours(g, s.aff, u, b, e, 1);
+      }
+    }
+    if (SYNC) append_moves(s, ws, moved, u, cb);
+    if (moved) {
+      ++moves;
+      if (!SYNC) ws.x += deg;
+    }
+  }
+  if (GW > 0) {
+    const unsigned long long gmask = __ballot(cls == 1);
+    if (gmask) moves += run_groups<SYNC, MG, UNIT, (GW > 0 ? GW : 4)>(a, gmask, u, b, e, d, pickless, ws);
+    const unsigned long long wmask = __ballot(cls == 2);
+    if (wmask) moves += run_groups<SYNC, MG, UNIT, 64>(a, wmask, u, b, e, d, pickless, ws);
+  }
+  ws.c += moves;
+  return (uint32_t)wave_sum_u64(moves);
+}
+
+// Persistent move kernel of sub-sweep `sub`: every wave takes `grab` consecutive items per
+// atomic pop until the queue is empty. Items are numbered graph after graph over the active list
+// (prefix[]), so a wave's items only move forward in it; item r of a graph is its row r.
+template <bool SYNC, bool MG, bool UNIT, int GW>
+__device__ __forceinline__ void batch_move(const BatchArgs &a, const BatchTune &t, uint32_t sub) {
+  if (a.s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t nitems = a.q->nitems;
+  const uint32_t nact = a.q->nact;
+  __shared__ BatchStage<UNIT> stage[kBlock / 64];
+  __shared__ uint64_t movebuf[SYNC ? kBlock / 64 : 1][SYNC ? kMoveBuf : 1];
+  BatchStage<UNIT> &L = stage[threadIdx.x >> 6];
+  WaveCtx ws{0, 0, 0, 0, SYNC ? movebuf[threadIdx.x >> 6] : nullptr, 0};
+  uint32_t k = 0;
+  uint32_t cur = 0xFFFFFFFFu;   // graph whose moves are being counted
+  unsigned long long cmoves = 0;
+  for (;;) {
+    unsigned long long i0 = 0;
+    if (lane == 0) i0 = atomicAdd(&a.q->head[sub], (unsigned long long)t.grab);
+    i0 = __shfl(i0, 0);
+    if (i0 >= nitems) break;
+    const uint64_t i1 = i0 + t.grab < nitems ? i0 + t.grab : nitems;
+    for (uint64_t it = i0; it < i1; ++it) {
+      if (!(a.prefix[k] <= it && it < a.prefix[k + 1])) {
+        uint32_t lo = k, hi = nact;  // largest lo with prefix[lo] <= it
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (a.prefix[mid] <= it) lo = mid;
+          else hi = mid;
+        }
+        k = lo;
+      }
+      const uint32_t id = a.active[k];
+      const uint32_t vb = a.slots[id].vb, n = a.slots[id].n, iter = a.slots[id].iter;
+      if (id != cur) {  // per-graph move counts: one atomic per run of items of a graph
+        if (lane == 0 && cmoves) atomicAdd(&a.slots[cur].changed, cmoves);
+        cur = id;
+        cmoves = 0;
+      }
+      cmoves += process_item<SYNC, MG, UNIT, GW>(a, t, vb, n, it - a.prefix[k], sub, iter, ws, L);
+    }
+  }
+  if (lane == 0 && cmoves) atomicAdd(&a.slots[cur].changed, cmoves);
+  if (SYNC) flush_moves(a.s, ws);
+  block_stats(a.bstats + 4ull * blockIdx.x, ws.a, ws.e, ws.c, ws.x);
+}
+
+// Lane-only classes: register use capped for 6 wavefronts per SIMD (more gathers in flight per
+// compute unit; this variant needs no spill at that cap).
+template <bool SYNC, bool MG, bool UNIT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_batch_move_lanes(BatchArgs a,
+                                                                                                   BatchTune t,
+                                                                                                   uint32_t sub) {
+  batch_move<SYNC, MG, UNIT, 0>(a, t, sub);
+}
+
+template <bool SYNC, bool MG, bool UNIT, int GW>
+__global__ __launch_bounds__(kBlock) void k_batch_move(BatchArgs a, BatchTune t, uint32_t sub) {
+  batch_move<SYNC, MG, UNIT, GW>(a, t, sub);
+}
+
+// Per running graph: the convergence test of rakLowmemLoopCuU (rakLowmemCuda.hxx:486-487) on
+// its own move count and iteration number.
+__global__ __launch_bounds__(kBlock) void k_batch_finish(BatchArgs a) {
+  if (a.s.ctr->done) return;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) a.s.ctr->iter += 1;
+  if (k >= a.q->nact) return;
+  BatchSlot &sl = a.slots[a.active[k]];
+  const uint32_t l = sl.iter;
+  const bool pickless = a.pickstep > 0 && (l % (uint32_t)a.pickstep) == 0;
+  const unsigned long long C = sl.changed;
+  sl.iter = l + 1;
+  sl.last_changed = C;
+  sl.changed = 0;
+  if ((!pickless && double(C) / double(sl.n) <= a.tolerance) || (int)(l + 1) >= a.max_iterations)
+    sl.state = kSlotDone;
+}
+
+// Active list and item prefix of the next step (one workgroup of 1024 lanes).
+constexpr int kPlanBlock = 1024;
+__global__ __launch_bounds__(kPlanBlock) void k_batch_plan(BatchArgs a) {
+  if (a.s.ctr->done) return;
+  __shared__ uint32_t s_cnt[kPlanBlock / 64];
+  __shared__ unsigned long long s_items[kPlanBlock / 64];
+  __shared__ uint32_t base_c;
+  __shared__ unsigned long long base_i;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  if (tid == 0) {
+    base_c = 0;
+    base_i = 0;
+    a.prefix[0] = 0;
+  }
+  __syncthreads();
+  for (uint32_t k0 = 0; k0 < a.nslots; k0 += kPlanBlock) {
+    const uint32_t id = k0 + (uint32_t)tid;
+    bool run = false;
+    unsigned long long items = 0;
+    if (id < a.nslots) {
+      const BatchSlot sl = a.slots[id];
+      run = sl.state == kSlotRunning;
+      items = run ? (sl.n + 64ull * a.sweeps - 1) / (64ull * a.sweeps) : 0ull;  // rows
+    }
+    const unsigned long long mask = __ballot(run);
+    unsigned long long incl = items;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long y = __shfl_up(incl, (unsigned)o);
+      if (lane >= o) incl += y;
+    }
+    if (lane == 63) {
+      s_cnt[wave] = (uint32_t)__popcll(mask);
+      s_items[wave] = incl;
+    }
+    __syncthreads();
+    uint32_t cb = base_c;
+    unsigned long long ib = base_i;
+    for (int w = 0; w < wave; ++w) {
+      cb += s_cnt[w];
+      ib += s_items[w];
+    }
+    if (run) {
+      const uint32_t pos = cb + lanes_below(mask);
+      a.active[pos] = id;
+      a.prefix[pos + 1] = ib + incl;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 0; w < kPlanBlock / 64; ++w) {
+        base_c += s_cnt[w];
+        base_i += s_items[w];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    for (int q = 0; q < kMaxSweeps; ++q) a.q->head[q] = 0;
+    a.q->nact = base_c;
+    a.q->nitems = base_i;
+    a.s.ctr->nmoves = 0;
+    if (base_c == 0) a.s.ctr->done = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_admit(BatchArgs a, const uint32_t *ids, const uint64_t *vfirst,
+                                                        uint32_t count, uint64_t total, const uint32_t *init,
+                                                        int reset) {
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t < count) {
+    BatchSlot &sl = a.slots[ids[t]];
+    sl.state = kSlotRunning;
+    sl.iter = 0;
+    sl.changed = 0;
+    sl.last_changed = 0;
+  }
+  if (t == 0) {
+    DevCounters *c = a.s.ctr;
+    c->done = 0;
+    c->nmoves = 0;
+    if (reset) {
+      c->A = c->E = c->C = c->X = 0;
+      c->iter = 0;
+    }
+  }
+  if (reset)
+    for (uint64_t i = t; i < 4ull * a.nbstats; i += (uint64_t)gridDim.x * kBlock) a.bstats[i] = 0;
+  uint32_t k = 0;
+  for (uint64_t p = t; p < total; p += (uint64_t)gridDim.x * kBlock) {
+    if (!(vfirst[k] <= p && p < vfirst[k + 1])) {
+      uint32_t lo = 0, hi = count;  // largest lo with vfirst[lo] <= p
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (vfirst[mid] <= p) lo = mid;
+        else hi = mid;
+      }
+      k = lo;
+    }
+    const uint32_t vb = a.slots[ids[k]].vb;
+    const uint32_t u = vb + (uint32_t)(p - vfirst[k]);
+    a.s.lab[u] = init ? init[u] + vb : u;
+    a.s.aff[u] = 1;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_rebase(uint64_t *off, uint64_t n, uint64_t eb, uint32_t *edg,
+                                                         uint64_t m, uint32_t vb, float *wei_ones) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i <= n; i += stride) off[i] += eb;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += stride) {
+    edg[i] += vb;
+    if (wei_ones) wei_ones[i] = 1.0f;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_set_state(BatchSlot *slots, const uint32_t *ids, uint32_t count,
+                                                            uint32_t state) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k < count) slots[ids[k]].state = state;
+}
+
+inline unsigned blocks_for(uint64_t items, uint64_t per_block, unsigned cap) {
+  uint64_t b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+template <bool SYNC, bool MG, bool UNIT>
+void launch_move_g(const BatchArgs &a, const BatchTune &t, hipStream_t st) {
+  const bool lanes_only = t.group_degree == 0xFFFFFFFFu && t.wave_degree == 0xFFFFFFFFu;
+  for (uint32_t sub = 0; sub < a.sweeps; ++sub) {
+    if (lanes_only) k_batch_move_lanes<SYNC, MG, UNIT><<<t.blocks, kBlock, 0, st>>>(a, t, sub);
+    else if (t.group_lanes == 4) k_batch_move<SYNC, MG, UNIT, 4><<<t.blocks, kBlock, 0, st>>>(a, t, sub);
+    else k_batch_move<SYNC, MG, UNIT, 16><<<t.blocks, kBlock, 0, st>>>(a, t, sub);
+  }
+}
+
+template <bool SYNC, bool MG>
+void launch_move_u(const BatchArgs &a, const BatchTune &t, hipStream_t st) {
+  if (a.g.wei) launch_move_g<SYNC, MG, false>(a, t, st);
+  else launch_move_g<SYNC, MG, true>(a, t, st);
+}
+
+}  // namespace
+
+void launch_batch_admit(const BatchArgs &a, const uint32_t *ids, const uint64_t *vfirst, uint32_t count,
+                        uint64_t total_vertices, const uint32_t *init, bool reset, hipStream_t st) {
+  const uint64_t items = total_vertices > count ? total_vertices : count;
+  k_batch_admit<<<blocks_for(items, kBlock, 8192), kBlock, 0, st>>>(a, ids, vfirst, count, total_vertices, init,
+                                                                     reset ? 1 : 0);
+}
+
+void launch_batch_rebase(uint64_t *off, uint64_t n, uint64_t eb, uint32_t *edg, uint64_t m, uint32_t vb,
+                         float *wei_ones, hipStream_t st) {
+  k_batch_rebase<<<blocks_for(m > n ? m : n + 1, kBlock, 8192), kBlock, 0, st>>>(off, n, eb, edg, m, vb, wei_ones);
+}
+
+void launch_batch_set_state(BatchSlot *slots, const uint32_t *ids, uint32_t count, uint32_t state, hipStream_t st) {
+  if (count) k_batch_set_state<<<blocks_for(count, kBlock, 65535), kBlock, 0, st>>>(slots, ids, count, state);
+}
+
+void launch_batch_plan(const BatchArgs &a, hipStream_t st) { k_batch_plan<<<1, kPlanBlock, 0, st>>>(a); }
+
+void launch_batch_step(const BatchArgs &a, const BatchTune &t, const BatchParams &p, hipStream_t st) {
+  if (p.sync) {
+    if (p.mg) launch_move_u<true, true>(a, t, st);
+    else launch_move_u<true, false>(a, t, st);
+    k_apply<<<512, kBlock, 0, st>>>(a.g, a.s);
+  } else {
+    if (p.mg) launch_move_u<false, true>(a, t, st);
+    else launch_move_u<false, false>(a, t, st);
+  }
+  k_batch_finish<<<blocks_for(a.nslots, kBlock, 65535), kBlock, 0, st>>>(a);
+  k_batch_plan<<<1, kPlanBlock, 0, st>>>(a);
+}
+
+}  // namespace rlc

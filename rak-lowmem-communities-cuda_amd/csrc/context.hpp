@@ -1,0 +1,46 @@
+// Internal pieces shared by the C ABI translation units: the context, HIP error plumbing,
+// device allocation and option validation.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "common.hpp"
+#include "rlc.h"
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return rlc::fail(RLC_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct rlc_context {
+  int device;
+};
+
+namespace rlc {
+
+template <class T>
+rlc_status dev_alloc(T **p, uint64_t count) {
+  size_t bytes = std::max<size_t>((size_t)count * sizeof(T), 256);
+  hipError_t e = hipMalloc((void **)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(e == hipErrorOutOfMemory ? RLC_ERR_OUT_OF_MEMORY : RLC_ERR_DEVICE,
+                std::string("hipMalloc(") + std::to_string(bytes) + " bytes): " + hipGetErrorString(e));
+  }
+  return RLC_OK;
+}
+
+// Validates every field of rlc_options (shared by rlc_run and rlc_batch_run).
+rlc_status check_options(const rlc_options *o);
+
+// Error state of the last kernel launch.
+inline rlc_status check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return RLC_OK;
+}
+
+}  // namespace rlc

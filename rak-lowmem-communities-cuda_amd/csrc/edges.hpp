@@ -1,0 +1,326 @@
+// Device helpers shared by the single-graph kernels and the batch kernels: edge streaming with
+// label gathers, neighbour marking, per-block counter flushes, and the synchronous apply pass.
+// Included by one translation unit each (everything lives in an anonymous namespace).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lpa.hpp"
+#include "sketch.hpp"
+
+namespace rlc {
+namespace {
+
+constexpr int kBlock = 256;
+
+
+// Per-block reduction of the four iteration counters into the device totals.
+struct BlockStats {
+  unsigned long long a, e, c, x;
+};
+
+__device__ __forceinline__ void flush_stats(DevCounters *ctr, unsigned long long a, unsigned long long e,
+                                            unsigned long long c, unsigned long long x) {
+  __shared__ unsigned long long red[4][kBlock / 64];
+  a = wave_sum_u64(a);
+  e = wave_sum_u64(e);
+  c = wave_sum_u64(c);
+  x = wave_sum_u64(x);
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) { red[0][wave] = a; red[1][wave] = e; red[2][wave] = c; red[3][wave] = x; }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
+    if (t) atomicAdd(&ctr->A + threadIdx.x, t);
+  }
+}
+
+__device__ __forceinline__ float edge_w(const DevGraph &g, uint64_t i) { return g.wei ? g.wei[i] : 1.0f; }
+
+constexpr int kBatch = 8;  // edges in flight per lane
+
+// Accesses to the streamed arrays (targets, weights, the label stream). Plain cached accesses:
+// non-temporal hints (__builtin_nontemporal_load/store) measured 15% slower on R-MAT 24.
+template <class T>
+__device__ __forceinline__ T stream_ld(const T *p) { return *p; }
+template <class T>
+__device__ __forceinline__ void stream_st(T *p, T v) { *p = v; }
+
+
+// Visit adjacency positions i, i+STRIDE, i+2*STRIDE, .. < e of vertex u in order, calling
+// f(label[v], w) for every target v != u. Positions are fetched kBatch at a time (targets and
+// weights, then the label gathers) so that each lane keeps kBatch independent loads in flight;
+// full batches are unpredicated, one predicated batch takes the tail. Indices are 32-bit
+// relative to the first position (a degree is below 2^32).
+template <bool UNIT, int STRIDE, class F, bool STORE = false>
+__device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t i, uint64_t e,
+                                          F f, uint32_t *lbuf = nullptr) {
+  if (i >= e) return;
+  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
+  const uint32_t *ep = g.edg + i;
+  const float *wp = UNIT ? nullptr : g.wei + i;
+  uint32_t *lp = STORE ? lbuf + i : nullptr;
+  uint32_t j = 0;
+  for (; j + kBatch <= cnt; j += kBatch) {
+    uint32_t vv[kBatch], cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      vv[k] = stream_ld(ep + (j + k) * STRIDE);
+      ww[k] = UNIT ? 1.0f : stream_ld(wp + (j + k) * STRIDE);
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));  // all loads issued before use
+    if (STORE) {
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) stream_st(lp + (j + k) * STRIDE, vv[k] != u ? cc[k] : kEmpty);
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (vv[k] != u) f(cc[k], ww[k]);
+  }
+  if (j < cnt) {
+    const uint32_t rem = cnt - j;
+    uint32_t vv[kBatch], cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint32_t jj = j + ((uint32_t)k < rem ? (uint32_t)k : 0u);
+      vv[k] = stream_ld(ep + jj * STRIDE);
+      ww[k] = UNIT ? 1.0f : stream_ld(wp + jj * STRIDE);
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));
+    if (STORE) {
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k)
+        if ((uint32_t)k < rem) stream_st(lp + (j + k) * STRIDE, vv[k] != u ? cc[k] : kEmpty);
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if ((uint32_t)k < rem && vv[k] != u) f(cc[k], ww[k]);
+  }
+}
+
+// The rescan's view of the same positions: (label, weight) streamed from the label buffer the
+// scan wrote (coalesced reads, no gather); self-loop positions hold kEmpty and are skipped.
+template <bool UNIT, int STRIDE, class F>
+__device__ __forceinline__ void for_labels(const DevGraph &g, const uint32_t *lbuf, uint64_t i, uint64_t e, F f) {
+  if (i >= e) return;
+  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
+  const uint32_t *lp = lbuf + i;
+  const float *wp = UNIT ? nullptr : g.wei + i;
+  uint32_t j = 0;
+  for (; j + kBatch <= cnt; j += kBatch) {
+    uint32_t cc[kBatch];
+    float ww[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      cc[k] = stream_ld(lp + (j + k) * STRIDE);
+      ww[k] = UNIT ? 1.0f : stream_ld(wp + (j + k) * STRIDE);
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (cc[k] != kEmpty) f(cc[k], ww[k]);
+  }
+  for (; j < cnt; ++j) {
+    const uint32_t c = stream_ld(lp + j * STRIDE);
+    const float w = UNIT ? 1.0f : stream_ld(wp + j * STRIDE);
+    if (c != kEmpty) f(c, w);
+  }
+}
+
+// aff[v] = 1 for targets v != u at positions i, i+stride, .. < e. The flag is read first and
+// written only when clear: the flags of the most connected vertices share a few cache lines
+// (degree-descending layout), and unconditional stores from every moved neighbour would
+// serialise on them.
+__device__ __forceinline__ void mark_neighbours(const DevGraph &g, uint8_t *aff, uint32_t u, uint64_t i, uint64_t e,
+                                                uint64_t stride) {
+  for (; i < e; i += stride * kBatch) {
+    uint32_t vv[kBatch];
+    uint8_t fl[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint64_t j = i + (uint64_t)k * stride;
+      vv[k] = j < e ? g.edg[j] : u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) fl[k] = aff[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (vv[k] != u && fl[k] == 0) aff[vv[k]] = 1;
+  }
+}
+
+__device__ __forceinline__ uint32_t range_of(const RangeSet &rs, uint64_t i) {
+  uint32_t lo = 0, hi = rs.count;  // acc[lo] <= i < acc[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (rs.acc[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- LDS-staged lane scan --------------------------------------------------------------------
+// Vertices scanned by one lane each (the sequential CSR-order sketch), with the memory work done
+// by the whole wavefront: the edges of the wave's vertices are laid end to end in an LDS buffer,
+// filled by the 64 lanes together (consecutive positions: coalesced target loads, label gathers
+// spread evenly over the lanes whatever the degrees), then every lane runs its own sketch and
+// rescan over its stretch of the buffer. The labels and weights a vertex sees, and the order
+// and values of its float additions, are those of for_edges, so the choice is the same.
+template <bool UNIT, int CAP>
+struct StageLds {
+  uint32_t lab[CAP];          // label of the target at each staged position (kEmpty: self-loop)
+  float w[UNIT ? 1 : CAP];    // its weight
+  uint64_t b[64];             // per lane: first edge position
+  uint32_t u[64];             // per lane: vertex
+  uint32_t start[64], end[64];  // per lane: its stretch [start, end) of the buffer this round
+};
+
+// LDS writes of a wave made visible to the wave's other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lanes with `part` set (their vertex u has edges [b, e) with e - b <= CAP) get the sketch's
+// choice (cb, wb); MG = Misra-Gries 8 slots with certified-or-rescanned argmax, else Boyer-Moore.
+// Called by the whole wave (all lanes, converged).
+template <bool MG, bool UNIT, int CAP>
+__device__ __forceinline__ void staged_choose(const DevGraph &g, const uint32_t *lab, StageLds<UNIT, CAP> &L, bool part,
+                                              uint32_t u, uint64_t b, uint64_t e, uint32_t &cb, float &wb) {
+  const int lane = lane_id();
+  const uint32_t deg = part ? (uint32_t)(e - b) : 0u;
+  uint32_t incl = deg;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t t = __shfl_up(incl, (unsigned)o);
+    if (lane >= o) incl += t;
+  }
+  const uint32_t excl = incl - deg;
+  L.b[lane] = b;
+  L.u[lane] = u;
+  unsigned long long pending = __ballot(part);
+  while (pending != 0ull) {
+    // this round: the longest run of pending lanes (in lane order) whose edges fit the buffer
+    const int first = __builtin_ctzll(pending);
+    const uint32_t base = __shfl(excl, first);
+    const bool in = ((pending >> lane) & 1ull) && incl - base <= (uint32_t)CAP;
+    const unsigned long long inmask = __ballot(in);
+    const int last = 63 - __builtin_clzll(inmask);
+    const uint32_t T = __shfl(incl, last) - base;
+    L.start[lane] = excl - base;
+    L.end[lane] = incl - base;
+    wave_lds_sync();
+    // fill: lane takes positions lane, lane + 64, ..; kBatch of them per round of loads
+    int i = first;
+    for (uint32_t p0 = (uint32_t)lane; p0 < T; p0 += 64u * kBatch) {
+      uint32_t vv[kBatch], cc[kBatch], uu[kBatch];
+      float ww[kBatch];
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const uint32_t p = p0 + 64u * (uint32_t)k;
+        vv[k] = 0;
+        uu[k] = kEmpty;
+        ww[k] = 1.0f;
+        if (p < T) {
+          while (L.end[i] <= p) ++i;
+          const uint64_t pos = L.b[i] + (p - L.start[i]);
+          uu[k] = L.u[i];
+          vv[k] = stream_ld(g.edg + pos);
+          if (!UNIT) ww[k] = stream_ld(g.wei + pos);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) cc[k] = p0 + 64u * (uint32_t)k < T ? lab[vv[k]] : kEmpty;
+#pragma unroll
+      for (int k = 0; k < kBatch; ++k) {
+        const uint32_t p = p0 + 64u * (uint32_t)k;
+        if (p < T) {
+          L.lab[p] = vv[k] == uu[k] ? kEmpty : cc[k];
+          if (!UNIT) L.w[p] = ww[k];
+        }
+      }
+    }
+    wave_lds_sync();
+    if (in) {
+      const uint32_t s0 = excl - base, s1 = incl - base;
+      if (MG) {
+        MG8 sk;
+        sk.clear();
+        for (uint32_t j = s0; j < s1; ++j) {
+          const uint32_t c = L.lab[j];
+          if (c != kEmpty) sk.accumulate(c, UNIT ? 1.0f : L.w[j]);
+        }
+        if (!certain_argmax(sk.c, sk.w, sk.dsum, (uint64_t)(s1 - s0), cb, wb)) {
+          float acc[kSlots];
+#pragma unroll
+          for (int q = 0; q < kSlots; ++q) acc[q] = 0.0f;
+          for (uint32_t j = s0; j < s1; ++j) {
+            const uint32_t c = L.lab[j];
+            const float w = UNIT ? 1.0f : L.w[j];
+            if (c != kEmpty) {
+#pragma unroll
+              for (int q = 0; q < kSlots; ++q) acc[q] = sk.c[q] == c ? acc[q] + w : acc[q];
+            }
+          }
+          argmax8(sk.c, acc, cb, wb);
+        }
+      } else {
+        uint32_t mc = kEmpty;
+        float mw = 0.0f;
+        for (uint32_t j = s0; j < s1; ++j) {
+          const uint32_t c = L.lab[j];
+          if (c != kEmpty) bm_step(mc, mw, c, UNIT ? 1.0f : L.w[j]);
+        }
+        cb = mc;
+        wb = mw;
+      }
+    }
+    wave_lds_sync();  // the buffer is refilled by the next round
+    pending &= ~inmask;
+  }
+}
+
+// ---- synchronous mode: commit the change list ------------------------------------------------
+// Lane per change for the label write; the wave then marks each change's neighbours together.
+__global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
+  if (s.ctr->done) return;
+  const unsigned long long nm = s.ctr->nmoves;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  unsigned long long sx = 0;
+  for (uint64_t base = wave_global * 64; base < nm; base += nwaves * 64) {
+    const uint64_t idx = base + lane;
+    uint32_t u = 0, c = 0;
+    uint64_t b = 0, e = 0;
+    if (idx < nm) {
+      const uint64_t mv = s.moves[idx];
+      u = (uint32_t)(mv >> 32);
+      c = (uint32_t)mv;
+      s.lab[u] = c;
+      b = g.off[u];
+      e = g.off[(uint64_t)u + 1];
+      sx += e - b;
+    }
+    const int cnt = (int)((nm - base) < 64 ? (nm - base) : 64);
+    for (int j = 0; j < cnt; ++j) {
+      const uint32_t uj = __shfl(u, j);
+      const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
+      mark_neighbours(g, s.aff, uj, bj + lane, ej, 64);
+    }
+  }
+  flush_stats(s.ctr, 0, 0, 0, sx);
+}
+
+}  // namespace
+}  // namespace rlc

@@ -13,9 +13,11 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
+#include "host.hpp"
 #include "rlc.h"
 
 namespace {
@@ -259,6 +261,83 @@ void rlc_csr_release(rlc_csr *g) {
   std::free(g->edges);
   std::free(g->weights);
   std::memset(g, 0, sizeof(*g));
+}
+
+}  // extern "C"
+
+extern "C" {
+
+rlc_status rlc_gen_sbm_set(uint32_t count, const uint64_t *n, const uint32_t *blocks, const double *degree,
+                           const double *intra, const uint64_t *seeds, rlc_csr_set *out) {
+  rlc::clear_error();
+  if (!out) return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_sbm_set: out is null");
+  std::memset(out, 0, sizeof(*out));
+  if (count && (!n || !blocks || !degree || !intra || !seeds))
+    return rlc::fail(RLC_ERR_INVALID_ARGUMENT, "rlc_gen_sbm_set: null parameter array");
+  // one graph per task, largest first; each graph's generator runs single-threaded inside the
+  // task (nested parallel regions get one thread), so its output is that of rlc_gen_sbm
+  std::vector<uint32_t> order(count);
+  for (uint32_t i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return n[x] > n[y]; });
+  std::vector<rlc_csr> parts(count);
+  std::vector<rlc_status> st(count, RLC_OK);
+  std::vector<std::string> msg(count);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t k = 0; k < (int64_t)count; ++k) {
+    const uint32_t i = order[(size_t)k];
+    std::memset(&parts[i], 0, sizeof(rlc_csr));
+    st[i] = rlc_gen_sbm(n[i], blocks[i], degree[i], intra[i], seeds[i], &parts[i]);
+    if (st[i] != RLC_OK) msg[i] = rlc::last_error_cstr();
+  }
+  auto release = [&]() {
+    for (auto &p : parts) rlc_csr_release(&p);
+  };
+  for (uint32_t i = 0; i < count; ++i)
+    if (st[i] != RLC_OK) {
+      release();
+      return rlc::fail(st[i], "rlc_gen_sbm_set: graph " + std::to_string(i) + ": " + msg[i]);
+    }
+  uint64_t *vs = (uint64_t *)std::malloc((count + 1) * sizeof(uint64_t));
+  uint64_t *es = (uint64_t *)std::malloc((count + 1) * sizeof(uint64_t));
+  if (!vs || !es) {
+    std::free(vs); std::free(es); release();
+    return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "rlc_gen_sbm_set: index arrays");
+  }
+  vs[0] = es[0] = 0;
+  for (uint32_t i = 0; i < count; ++i) {
+    vs[i + 1] = vs[i] + parts[i].n;
+    es[i + 1] = es[i] + parts[i].m;
+  }
+  uint64_t *off = (uint64_t *)std::malloc((vs[count] + count + 1) * sizeof(uint64_t));
+  uint32_t *edg = (uint32_t *)std::malloc(std::max<uint64_t>(es[count], 1) * sizeof(uint32_t));
+  if (!off || !edg) {
+    std::free(vs); std::free(es); std::free(off); std::free(edg); release();
+    return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "rlc_gen_sbm_set: output arrays");
+  }
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t k = 0; k < (int64_t)count; ++k) {
+    const uint32_t i = order[(size_t)k];
+    std::memcpy(off + vs[i] + i, parts[i].offsets, (parts[i].n + 1) * sizeof(uint64_t));
+    if (parts[i].m) std::memcpy(edg + es[i], parts[i].edges, parts[i].m * sizeof(uint32_t));
+    rlc_csr_release(&parts[i]);
+  }
+  out->count = count;
+  out->vstart = vs;
+  out->estart = es;
+  out->offsets = off;
+  out->edges = edg;
+  out->weights = nullptr;
+  return RLC_OK;
+}
+
+void rlc_csr_set_release(rlc_csr_set *s) {
+  if (!s) return;
+  std::free(s->vstart);
+  std::free(s->estart);
+  std::free(s->offsets);
+  std::free(s->edges);
+  std::free(s->weights);
+  std::memset(s, 0, sizeof(*s));
 }
 
 }  // extern "C"

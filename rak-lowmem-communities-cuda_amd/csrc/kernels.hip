@@ -22,168 +22,13 @@
 
 #include <cstdint>
 
+#include "edges.hpp"
 #include "lpa.hpp"
 #include "sketch.hpp"
 
 namespace rlc {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-
-// Per-block reduction of the four iteration counters into the device totals.
-struct BlockStats {
-  unsigned long long a, e, c, x;
-};
-
-__device__ __forceinline__ void flush_stats(DevCounters *ctr, unsigned long long a, unsigned long long e,
-                                            unsigned long long c, unsigned long long x) {
-  __shared__ unsigned long long red[4][kBlock / 64];
-  a = wave_sum_u64(a);
-  e = wave_sum_u64(e);
-  c = wave_sum_u64(c);
-  x = wave_sum_u64(x);
-  const int lane = lane_id(), wave = threadIdx.x >> 6;
-  if (lane == 0) { red[0][wave] = a; red[1][wave] = e; red[2][wave] = c; red[3][wave] = x; }
-  __syncthreads();
-  if (threadIdx.x < 4) {
-    unsigned long long t = 0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[threadIdx.x][w];
-    if (t) atomicAdd(&ctr->A + threadIdx.x, t);
-  }
-}
-
-__device__ __forceinline__ float edge_w(const DevGraph &g, uint64_t i) { return g.wei ? g.wei[i] : 1.0f; }
-
-constexpr int kBatch = 8;  // edges in flight per lane
-
-// Accesses to the streamed arrays (targets, weights, the label stream). Plain cached accesses:
-// non-temporal hints (__builtin_nontemporal_load/store) measured 15% slower on R-MAT 24.
-template <class T>
-__device__ __forceinline__ T stream_ld(const T *p) { return *p; }
-template <class T>
-__device__ __forceinline__ void stream_st(T *p, T v) { *p = v; }
-
-
-// Visit adjacency positions i, i+STRIDE, i+2*STRIDE, .. < e of vertex u in order, calling
-// f(label[v], w) for every target v != u. Positions are fetched kBatch at a time (targets and
-// weights, then the label gathers) so that each lane keeps kBatch independent loads in flight;
-// full batches are unpredicated, one predicated batch takes the tail. Indices are 32-bit
-// relative to the first position (a degree is below 2^32).
-template <bool UNIT, int STRIDE, class F, bool STORE = false>
-__device__ __forceinline__ void for_edges(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t i, uint64_t e,
-                                          F f, uint32_t *lbuf = nullptr) {
-  if (i >= e) return;
-  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
-  const uint32_t *ep = g.edg + i;
-  const float *wp = UNIT ? nullptr : g.wei + i;
-  uint32_t *lp = STORE ? lbuf + i : nullptr;
-  uint32_t j = 0;
-  for (; j + kBatch <= cnt; j += kBatch) {
-    uint32_t vv[kBatch], cc[kBatch];
-    float ww[kBatch];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
-      vv[k] = stream_ld(ep + (j + k) * STRIDE);
-      ww[k] = UNIT ? 1.0f : stream_ld(wp + (j + k) * STRIDE);
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));  // all loads issued before use
-    if (STORE) {
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k) stream_st(lp + (j + k) * STRIDE, vv[k] != u ? cc[k] : kEmpty);
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k)
-      if (vv[k] != u) f(cc[k], ww[k]);
-  }
-  if (j < cnt) {
-    const uint32_t rem = cnt - j;
-    uint32_t vv[kBatch], cc[kBatch];
-    float ww[kBatch];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
-      const uint32_t jj = j + ((uint32_t)k < rem ? (uint32_t)k : 0u);
-      vv[k] = stream_ld(ep + jj * STRIDE);
-      ww[k] = UNIT ? 1.0f : stream_ld(wp + jj * STRIDE);
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) cc[k] = lab[vv[k]];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) asm volatile("" ::"v"(cc[k]), "v"(ww[k]));
-    if (STORE) {
-#pragma unroll
-      for (int k = 0; k < kBatch; ++k)
-        if ((uint32_t)k < rem) stream_st(lp + (j + k) * STRIDE, vv[k] != u ? cc[k] : kEmpty);
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k)
-      if ((uint32_t)k < rem && vv[k] != u) f(cc[k], ww[k]);
-  }
-}
-
-// The rescan's view of the same positions: (label, weight) streamed from the label buffer the
-// scan wrote (coalesced reads, no gather); self-loop positions hold kEmpty and are skipped.
-template <bool UNIT, int STRIDE, class F>
-__device__ __forceinline__ void for_labels(const DevGraph &g, const uint32_t *lbuf, uint64_t i, uint64_t e, F f) {
-  if (i >= e) return;
-  const uint32_t cnt = (uint32_t)((e - i + STRIDE - 1) / STRIDE);
-  const uint32_t *lp = lbuf + i;
-  const float *wp = UNIT ? nullptr : g.wei + i;
-  uint32_t j = 0;
-  for (; j + kBatch <= cnt; j += kBatch) {
-    uint32_t cc[kBatch];
-    float ww[kBatch];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
-      cc[k] = stream_ld(lp + (j + k) * STRIDE);
-      ww[k] = UNIT ? 1.0f : stream_ld(wp + (j + k) * STRIDE);
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k)
-      if (cc[k] != kEmpty) f(cc[k], ww[k]);
-  }
-  for (; j < cnt; ++j) {
-    const uint32_t c = stream_ld(lp + j * STRIDE);
-    const float w = UNIT ? 1.0f : stream_ld(wp + j * STRIDE);
-    if (c != kEmpty) f(c, w);
-  }
-}
-
-// aff[v] = 1 for targets v != u at positions i, i+stride, .. < e. The flag is read first and
-// written only when clear: the flags of the most connected vertices share a few cache lines
-// (degree-descending layout), and unconditional stores from every moved neighbour would
-// serialise on them.
-__device__ __forceinline__ void mark_neighbours(const DevGraph &g, uint8_t *aff, uint32_t u, uint64_t i, uint64_t e,
-                                                uint64_t stride) {
-  for (; i < e; i += stride * kBatch) {
-    uint32_t vv[kBatch];
-    uint8_t fl[kBatch];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) {
-      const uint64_t j = i + (uint64_t)k * stride;
-      vv[k] = j < e ? g.edg[j] : u;
-    }
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k) fl[k] = aff[vv[k]];
-#pragma unroll
-    for (int k = 0; k < kBatch; ++k)
-      if (vv[k] != u && fl[k] == 0) aff[vv[k]] = 1;
-  }
-}
-
-__device__ __forceinline__ uint32_t range_of(const RangeSet &rs, uint64_t i) {
-  uint32_t lo = 0, hi = rs.count;  // acc[lo] <= i < acc[hi]
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (rs.acc[mid] <= i) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // ---- lane per vertex: one sequential sketch per lane ----------------------------------------
 // Items are 64-position chunks of the ranges of `rs`; one chunk per wave.
@@ -650,38 +495,6 @@ __global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, C
     const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
     mark_neighbours(g, s.aff, u, b + lane, e, 64);
   }
-}
-
-// ---- synchronous mode: commit the change list ------------------------------------------------
-// Lane per change for the label write; the wave then marks each change's neighbours together.
-__global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
-  if (s.ctr->done) return;
-  const unsigned long long nm = s.ctr->nmoves;
-  const int lane = lane_id();
-  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
-  unsigned long long sx = 0;
-  for (uint64_t base = wave_global * 64; base < nm; base += nwaves * 64) {
-    const uint64_t idx = base + lane;
-    uint32_t u = 0, c = 0;
-    uint64_t b = 0, e = 0;
-    if (idx < nm) {
-      const uint64_t mv = s.moves[idx];
-      u = (uint32_t)(mv >> 32);
-      c = (uint32_t)mv;
-      s.lab[u] = c;
-      b = g.off[u];
-      e = g.off[(uint64_t)u + 1];
-      sx += e - b;
-    }
-    const int cnt = (int)((nm - base) < 64 ? (nm - base) : 64);
-    for (int j = 0; j < cnt; ++j) {
-      const uint32_t uj = __shfl(u, j);
-      const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
-      mark_neighbours(g, s.aff, uj, bj + lane, ej, 64);
-    }
-  }
-  flush_stats(s.ctr, 0, 0, 0, sx);
 }
 
 // ---- end of an iteration: convergence test on the device ------------------------------------

@@ -68,14 +68,25 @@ struct MG8 {
   }
 };
 
+// Running maximum over slots I.. with compile-time indices: the selects pick register values
+// (a loop form can be rewritten by the compiler into an indexed load from a stack copy).
+template <int I>
+__device__ __forceinline__ void max_from(const uint32_t (&c)[kSlots], const float (&w)[kSlots], uint32_t &cb,
+                                         float &wb, int &top) {
+  const bool gt = w[I] > wb;
+  cb = gt ? c[I] : cb;
+  wb = gt ? w[I] : wb;
+  top = gt ? I : top;
+  if constexpr (I + 1 < kSlots) max_from<I + 1>(c, w, cb, wb, top);
+}
+
 // First slot of strictly largest weight, from (label 0, weight 0) (inc/rakLowmem.hxx:164-171).
 __device__ __forceinline__ void argmax8(const uint32_t (&c)[kSlots], const float (&w)[kSlots], uint32_t &cb,
                                         float &wb) {
   cb = 0;
   wb = 0.0f;
-#pragma unroll
-  for (int i = 0; i < kSlots; ++i)
-    if (w[i] > wb) { wb = w[i]; cb = c[i]; }
+  int top = 0;
+  max_from<0>(c, w, cb, wb, top);
 }
 
 // The reference's choice after a rescan, when it is already determined by the sketch: with no
@@ -88,17 +99,19 @@ __device__ __forceinline__ bool certain_argmax(const uint32_t (&c)[kSlots], cons
     argmax8(c, w, cb, wb);
     return true;
   }
+  // first heaviest slot (weights are >= 0 and D > 0 here, so some slot is non-zero or the
+  // test below fails) and the largest other weight
   int top = 0;
-#pragma unroll
-  for (int i = 1; i < kSlots; ++i) top = w[i] > w[top] ? i : top;
+  float wt = w[0];
+  uint32_t ct = c[0];
+  max_from<1>(c, w, ct, wt, top);
   float second = 0.0f;
 #pragma unroll
   for (int i = 0; i < kSlots; ++i) second = (i != top && w[i] > second) ? w[i] : second;
-  const float eps = (float)(deg + 8) * 0x1.0p-21f * (w[top] + D);
-  if (!(w[top] > second + D + eps) || !(w[top] > eps)) return false;
-#pragma unroll
-  for (int i = 0; i < kSlots; ++i)
-    if (i == top) { cb = c[i]; wb = w[i]; }
+  const float eps = (float)(deg + 8) * 0x1.0p-21f * (wt + D);
+  if (!(wt > second + D + eps) || !(wt > eps)) return false;
+  cb = ct;
+  wb = wt;
   return true;
 }
 

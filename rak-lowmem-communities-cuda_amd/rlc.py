@@ -63,6 +63,27 @@ class Csr(C.Structure):
                 ("edges", C.POINTER(C.c_uint32)), ("weights", C.POINTER(C.c_float))]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("blocks_per_cu", C.c_int32), ("grab", C.c_int32), ("group_degree", C.c_int32),
+                ("group_lanes", C.c_int32), ("wave_degree", C.c_int32), ("interleave", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("graphs", C.c_uint32), ("steps", C.c_int32), ("time_ms", C.c_float), ("reserved_f", C.c_float),
+                ("vertices", C.c_uint64), ("edges", C.c_uint64), ("traversed_edges", C.c_uint64),
+                ("changed", C.c_uint64)]
+
+
+class CsrSet(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("vstart", C.POINTER(C.c_uint64)), ("estart", C.POINTER(C.c_uint64)),
+                ("offsets", C.POINTER(C.c_uint64)), ("edges", C.POINTER(C.c_uint32)),
+                ("weights", C.POINTER(C.c_float))]
+
+
 # (name, restype, argtypes) of every exported entry point, in include/rlc.h order
 _P = C.c_void_p
 SIGNATURES = [
@@ -81,11 +102,23 @@ SIGNATURES = [
     ("rlc_graph_labels_device", _P, [_P]),
     ("rlc_graph_vertex_order", C.c_int, [_P, _P]),
     ("rlc_modularity", C.c_int, [_P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    ("rlc_batch_tuning_default", None, [C.POINTER(BatchTuning)]),
+    ("rlc_batch_create", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int32, C.POINTER(_P)]),
+    ("rlc_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int32, C.c_int32]),
+    ("rlc_batch_add", C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, C.POINTER(C.c_uint32)]),
+    ("rlc_batch_remove", C.c_int, [_P, C.c_uint32]),
+    ("rlc_batch_run", C.c_int, [_P, _P, C.c_uint32, C.POINTER(Options), C.POINTER(BatchTuning),
+                                C.POINTER(BatchStats)]),
+    ("rlc_batch_result", C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_int32)]),
+    ("rlc_batch_tune", C.c_int, [_P, _P, C.c_uint32, C.POINTER(Options), C.POINTER(BatchTuning)]),
+    ("rlc_batch_destroy", None, [_P]),
     ("rlc_gen_sbm", C.c_int, [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Csr)]),
     ("rlc_gen_rmat", C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_float,
                                C.c_float, C.c_uint64, C.POINTER(Csr)]),
     ("rlc_gen_grid", C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Csr)]),
     ("rlc_csr_release", None, [C.POINTER(Csr)]),
+    ("rlc_gen_sbm_set", C.c_int, [C.c_uint32, _P, _P, _P, _P, _P, C.POINTER(CsrSet)]),
+    ("rlc_csr_set_release", None, [C.POINTER(CsrSet)]),
 ]
 
 _lib = None
@@ -170,6 +203,53 @@ def gen_grid(rows: int, cols: int, drop: float, shortcut: float, seed: int) -> G
     return _take_csr(c, unit=True)
 
 
+class GraphSet:
+    """Many host CSRs in shared buffers (rlc_csr_set); graph(i) returns views, local ids."""
+
+    def __init__(self, vstart, estart, offsets, edges):
+        self.vstart, self.estart, self.offsets, self.edges = vstart, estart, offsets, edges
+
+    def __len__(self):
+        return len(self.vstart) - 1
+
+    def graph(self, i: int) -> Graph:
+        v0, v1 = int(self.vstart[i]), int(self.vstart[i + 1])
+        e0, e1 = int(self.estart[i]), int(self.estart[i + 1])
+        return Graph(self.offsets[v0 + i: v1 + i + 1], self.edges[e0:e1], None)
+
+
+def gen_sbm_set(n, blocks, degree, intra, seeds) -> GraphSet:
+    """Graph i = gen_sbm(n[i], blocks[i], degree[i], intra[i], seeds[i]), generated in parallel."""
+    n = np.ascontiguousarray(n, np.uint64)
+    blocks = np.ascontiguousarray(blocks, np.uint32)
+    degree = np.ascontiguousarray(degree, np.float64)
+    intra = np.ascontiguousarray(intra, np.float64)
+    seeds = np.ascontiguousarray(seeds, np.uint64)
+    c = CsrSet()
+    _check(lib().rlc_gen_sbm_set(len(n), _ptr(n), _ptr(blocks), _ptr(degree), _ptr(intra), _ptr(seeds),
+                                 C.byref(c)))
+    k = int(c.count)
+    vs = np.ctypeslib.as_array(c.vstart, shape=(k + 1,)).copy()
+    es = np.ctypeslib.as_array(c.estart, shape=(k + 1,)).copy()
+    off = np.ctypeslib.as_array(c.offsets, shape=(int(vs[-1]) + k,)).copy() if k else np.zeros(0, np.uint64)
+    edg = np.ctypeslib.as_array(c.edges, shape=(int(es[-1]),)).copy() if k and es[-1] else np.zeros(0, np.uint32)
+    lib().rlc_csr_set_release(C.byref(c))
+    return GraphSet(vs, es, off, edg)
+
+
+def sbm_mix_params(count: int, seed: int, n_min: int = 1000, n_max: int = 1_000_000):
+    """Parameters of the batched workload's graph mix: vertex counts log-uniform in
+    [n_min, n_max], 4..1024 blocks (at most n/2), expected degree uniform in [4, 32], 70-90% of
+    edges inside blocks, per-graph seeds; drawn from one seeded numpy generator."""
+    r = np.random.default_rng(seed)
+    n = np.exp(r.uniform(np.log(n_min), np.log(n_max), count)).astype(np.uint64)
+    blocks = np.minimum(r.integers(4, 1025, count), n // 2).astype(np.uint32)
+    degree = r.uniform(4.0, 32.0, count)
+    intra = r.uniform(0.7, 0.9, count)
+    seeds = r.integers(1, 2**63 - 1, count, dtype=np.int64).astype(np.uint64)
+    return n, blocks, degree, intra, seeds
+
+
 def options(**kw) -> Options:
     o = Options()
     lib().rlc_options_default(C.byref(o))
@@ -227,6 +307,87 @@ class Context:
             off = np.ascontiguousarray(g.offsets, dtype=np.uint64)
             _check(lib().rlc_graph_upload(self.h, g.n, g.m, _ptr(off), _ptr(edg), _ptr(wei), C.byref(h)))
         return DeviceGraph(self, h, g.n)
+
+
+    def batch(self, vertex_capacity: int, edge_capacity: int, max_graphs: int, weighted: bool = False) -> "Batch":
+        h = C.c_void_p()
+        _check(lib().rlc_batch_create(self.h, vertex_capacity, edge_capacity, max_graphs, 1 if weighted else 0,
+                                      C.byref(h)))
+        return Batch(self, h)
+
+
+def tuning(**kw) -> BatchTuning:
+    t = BatchTuning()
+    lib().rlc_batch_tuning_default(C.byref(t))
+    for k, v in kw.items():
+        if not hasattr(t, k):
+            raise KeyError(k)
+        setattr(t, k, v)
+    return t
+
+
+def batch_footprint(n: int, m: int, weighted: bool = False, sync: bool = False) -> int:
+    return int(lib().rlc_batch_footprint(n, m, 1 if weighted else 0, 1 if sync else 0))
+
+
+class Batch:
+    """Many graphs resident in one device arena (rlc_batch_*)."""
+
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+        self.sizes = {}
+
+    def close(self):
+        if self.h:
+            lib().rlc_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, g: Graph) -> int:
+        gid = C.c_uint32()
+        off = np.ascontiguousarray(g.offsets, dtype=np.uint64)
+        edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
+        wei = None if g.weights is None else np.ascontiguousarray(g.weights, dtype=np.float32)
+        _check(lib().rlc_batch_add(self.h, g.n, g.m, _ptr(off), _ptr(edg), _ptr(wei), C.byref(gid)))
+        self.sizes[gid.value] = g.n
+        return gid.value
+
+    def remove(self, gid: int) -> None:
+        _check(lib().rlc_batch_remove(self.h, gid))
+        self.sizes.pop(gid, None)
+
+    @staticmethod
+    def _ids(ids):
+        if ids is None:
+            return None, 0
+        a = np.ascontiguousarray(ids, dtype=np.uint32)
+        return a, len(a)
+
+    def run(self, opts: Options | None = None, ids=None, tune: BatchTuning | None = None) -> BatchStats:
+        o = opts if opts is not None else options()
+        a, k = self._ids(ids)
+        st = BatchStats()
+        _check(lib().rlc_batch_run(self.h, _ptr(a), k, C.byref(o), None if tune is None else C.byref(tune),
+                                   C.byref(st)))
+        return st
+
+    def result(self, gid: int):
+        lab = np.empty(self.sizes[gid], np.uint32)
+        it = C.c_int32()
+        _check(lib().rlc_batch_result(self.h, gid, _ptr(lab), C.byref(it)))
+        return lab, it.value
+
+    def tune(self, opts: Options | None = None, ids=None) -> BatchTuning:
+        o = opts if opts is not None else options()
+        a, k = self._ids(ids)
+        t = BatchTuning()
+        _check(lib().rlc_batch_tune(self.h, _ptr(a), k, C.byref(o), C.byref(t)))
+        return t
 
 
 class DeviceGraph:
