@@ -197,6 +197,95 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+def serve_stream(ctx, rlc, graphs, arrivals, opts, cfg):
+    """Submit graphs[i] at arrivals[i] (seconds from the start) to a fresh server; returns the
+    latencies (ms, submit to labels on the host), the makespan (s) and the server's stats."""
+    with rlc.Server(ctx, cfg, opts) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for g, t in zip(graphs, arrivals):
+            while True:
+                dt = t - (time.perf_counter() - t0)
+                if dt <= 0:
+                    break
+                time.sleep(min(dt, 0.002))
+            ids.append(srv.submit(g))
+        srv.drain()
+        infos = [srv.result(r)[1] for r in ids]
+        st = srv.stats()
+    lat = [i.done_ms - i.submit_ms for i in infos]
+    span = (max(i.done_ms for i in infos) - min(i.submit_ms for i in infos)) / 1e3
+    return lat, span, st, infos
+
+
+def bench_serve(args):
+    """Serving workload: a Poisson stream of requests, graphs drawn from the batched workload's
+    SBM distribution with 1% R-MAT scale 22 graphs, MG8 asynchronous; continuous batching vs
+    one graph resident at a time, on the same arrival times."""
+    import numpy as np
+
+    import rlc
+
+    N = args.requests
+    P = rlc.sbm_mix_params(N, seed=args.batch_seed + 100)
+    t0 = time.time()
+    S = rlc.gen_sbm_set(*P)
+    graphs = [S.graph(i) for i in range(N)]
+    rmat = [rlc.gen_rmat(22, 16, 0.57, 0.19, 0.19, 1.0, 1.0, seed=s) for s in (1, 2)]
+    n_rmat = 0
+    for i in range(0, N, 100):  # 1% of the requests
+        graphs[i] = rmat[(i // 100) % 2]
+        n_rmat += 1
+    log(f"serve: {N} requests generated ({time.time() - t0:.0f}s), {n_rmat} R-MAT 22 "
+        f"(n={rmat[0].n}, m={rmat[0].m}), SBM entries {int(S.estart[-1])}")
+    opts = rlc.options(slots=args.slots)  # asynchronous
+    budget = int(args.serve_gb * (1 << 30))
+    # arena split: ~20 entries per vertex entry (the mix has degree 4-32, R-MAT 22 about 30)
+    V = budget // (13 + 20 * 4)
+    E = 20 * V
+    cfg = rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=4096, steps_per_round=args.round_steps)
+    ctx = rlc.Context(0)
+    # capacity: a closed burst of the first requests
+    burst = graphs[: min(N, 300)]
+    lat_b, span_b, _, _ = serve_stream(ctx, rlc, burst, [0.0] * len(burst), opts, cfg)
+    cap = len(burst) / span_b
+    lam = args.load * cap
+    arrivals = np.cumsum(np.random.default_rng(args.batch_seed).exponential(1.0 / lam, N))
+    log(f"serve: burst capacity {cap:.1f} graphs/s; Poisson rate {lam:.1f}/s ({args.load:.0%} of it)")
+    lat, span, st, infos = serve_stream(ctx, rlc, graphs, arrivals, opts, cfg)
+    log(f"continuous batching: makespan {span:.2f}s, rounds {st.rounds}, resident max {st.resident_max}")
+    cfg1 = rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=1, steps_per_round=args.round_steps)
+    lat1, span1, st1, _ = serve_stream(ctx, rlc, graphs, arrivals, opts, cfg1)
+    log(f"one at a time: makespan {span1:.2f}s")
+    ctx.close()
+    p = lambda x, q: float(np.percentile(np.array(x), q))
+    line = {
+        "metric": "p99 latency (serving)",
+        "value": p(lat, 99),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": N,
+        "warmup": len(burst),
+        "ms_per_step": span * 1e3 / N,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{N} requests, Poisson arrivals at {lam:.1f}/s ({args.load:.0%} of the burst capacity), "
+                        f"SBM graphs n log-uniform [1e3, 1e6] + 1% R-MAT scale 22, MG8 asynchronous, "
+                        f"{args.serve_gb} GiB arena, {args.round_steps} iterations per round",
+            "p50_ms": p(lat, 50), "p99_ms": p(lat, 99), "graphs_per_s": N / span,
+            "one_at_a_time": {"p50_ms": p(lat1, 50), "p99_ms": p(lat1, 99), "graphs_per_s": N / span1},
+            "burst_capacity_graphs_per_s": cap, "rate_per_s": lam, "resident_max": st.resident_max,
+            "rounds": st.rounds, "rmat22_requests": n_rmat,
+            "rmat22_p50_ms": p([l for l, i in zip(lat, infos) if i.n == rmat[0].n], 50),
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,7 +297,11 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
-    ap.add_argument("--workload", choices=["rmat", "batch"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "batch", "serve"], default="rmat")
+    ap.add_argument("--requests", type=int, default=1000, help="serve workload: requests")
+    ap.add_argument("--load", type=float, default=0.8, help="serve workload: arrival rate / burst capacity")
+    ap.add_argument("--serve-gb", type=float, default=24.0, help="serve workload: arena size (GiB)")
+    ap.add_argument("--round-steps", type=int, default=2, help="serve workload: iterations per round")
     ap.add_argument("--graphs", type=int, default=4096, help="batch workload: number of graphs")
     ap.add_argument("--batch-seed", type=int, default=1)
     ap.add_argument("--run-sample", type=int, default=256,
@@ -224,6 +317,9 @@ def main():
 
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
 
     import numpy as np

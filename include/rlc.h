@@ -240,6 +240,65 @@ rlc_status rlc_batch_tune(rlc_batch *b, const uint32_t *ids, uint32_t count, con
                           rlc_batch_tuning *best);
 void rlc_batch_destroy(rlc_batch *b);
 
+/* ---- serving: continuous batching on a resident arena ---------------------------------------
+ * No counterpart in the reference. A worker thread owns an arena (as rlc_batch) and runs rounds:
+ * it launches steps_per_round iterations of every resident graph, uploads newly admitted graphs
+ * on a second stream while they run, then retires the graphs that finished (labels copied to
+ * the host, arena space freed) and admits queued requests in arrival order while they fit (a
+ * free graph slot and free arena ranges: admission by HBM footprint). A request that arrives
+ * while others run joins the next round; each graph keeps its own iteration count and
+ * convergence test, so its labels do not depend on the mix in RLC_MODE_SYNC. */
+typedef struct rlc_server rlc_server;
+
+typedef struct rlc_server_config {
+  uint64_t vertex_capacity; /* arena vertex entries (a graph of n vertices takes n + 1) */
+  uint64_t edge_capacity;   /* arena directed entries */
+  uint32_t max_graphs;      /* resident graphs at most */
+  int32_t weighted;         /* the arena stores weights (else every request must be unweighted) */
+  int32_t steps_per_round;  /* iterations launched between two admission / retirement points [2] */
+  int32_t reserved[3];      /* must be zero */
+} rlc_server_config;
+
+typedef struct rlc_request_info {
+  int32_t status;     /* RLC_OK, or why the request failed */
+  int32_t iterations; /* iterations of its run */
+  uint64_t n, m;
+  double submit_ms;   /* server clock (rlc_server_now_ms) when submitted */
+  double admit_ms;    /* when its copy into the arena started */
+  double done_ms;     /* when its labels were on the host */
+} rlc_request_info;
+
+typedef struct rlc_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t rounds, steps;     /* rounds run, iterations launched */
+  uint64_t traversed_edges;   /* over all requests and iterations */
+  uint32_t resident_max;      /* most graphs resident at once */
+  uint32_t reserved;
+} rlc_server_stats;
+
+void rlc_server_config_default(rlc_server_config *c);
+/* Start a server: arena of c's capacities on ctx's device, every request run with options o
+ * (repeat and timing ignored) and tuning t (NULL: defaults). */
+rlc_status rlc_server_create(rlc_context *ctx, const rlc_server_config *c, const rlc_options *o,
+                             const rlc_batch_tuning *t, rlc_server **out);
+/* Queue a request (same CSR rules as rlc_graph_upload; validated here). The arrays are read when
+ * the request is admitted and must stay valid until rlc_server_result reports it finished.
+ * RLC_ERR_CAPACITY when the graph can never fit the arena. */
+rlc_status rlc_server_submit(rlc_server *s, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
+                             const float *weights, uint64_t *request);
+/* A finished request's labels (n entries, local ids; labels may be NULL) and info; the request is
+ * then forgotten. RLC_ERR_NOT_READY while it is queued or running; the request's own error
+ * status if it failed. */
+rlc_status rlc_server_result(rlc_server *s, uint64_t request, uint32_t *labels, rlc_request_info *info);
+/* Wait until every submitted request has finished; timeout_ms < 0 waits without limit.
+ * RLC_ERR_NOT_READY on timeout. */
+rlc_status rlc_server_drain(rlc_server *s, int64_t timeout_ms);
+rlc_status rlc_server_stats_get(rlc_server *s, rlc_server_stats *st);
+/* The server's clock: milliseconds since rlc_server_create (host steady clock). */
+double rlc_server_now_ms(const rlc_server *s);
+/* Stop the worker (requests not finished are dropped) and free the arena. */
+void rlc_server_destroy(rlc_server *s);
+
 /* ---- synthetic inputs (host, seeded, deterministic for any thread count) ------------------- */
 typedef struct rlc_csr {
   uint64_t n, m;

@@ -14,109 +14,21 @@
 
 #include "arena.hpp"
 #include "batch.hpp"
+#include "batch_host.hpp"
 #include "context.hpp"
 #include "host.hpp"
 
 using namespace rlc;
 
-namespace {
+rlc_status rlc::ensure_moves(rlc_batch *b, const rlc_options &o) {
+  if (o.mode != RLC_MODE_SYNC || b->d_moves) return RLC_OK;
+  rlc_status st = dev_alloc(&b->d_moves, b->vcap);
+  if (st != RLC_OK) return st;
+  b->key.valid = false;  // captured launches embed the state pointers
+  return RLC_OK;
+}
 
-constexpr uint32_t kNever = 0xFFFFFFFFu;
-
-struct BatchKey {
-  bool valid = false;
-  int sync = 0, mg = 0, pickstep = 0, max_iterations = 0, sweeps = 0, interleave = 0;
-  double tolerance = 0;
-  BatchTune t{};
-  bool operator==(const BatchKey &o) const {
-    return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
-           max_iterations == o.max_iterations && sweeps == o.sweeps && interleave == o.interleave &&
-           tolerance == o.tolerance &&
-           t.blocks == o.t.blocks &&
-           t.grab == o.t.grab && t.group_degree == o.t.group_degree && t.group_lanes == o.t.group_lanes &&
-           t.wave_degree == o.t.wave_degree;
-  }
-};
-
-struct HostSlot {
-  bool used = false;
-  uint64_t vb = 0, n = 0, eb = 0, m = 0;
-};
-
-struct BusyFlag {
-  std::atomic<int> &b;
-  bool ok;
-  explicit BusyFlag(std::atomic<int> &x) : b(x) {
-    int z = 0;
-    ok = b.compare_exchange_strong(z, 1);
-  }
-  ~BusyFlag() {
-    if (ok) b.store(0);
-  }
-};
-
-}  // namespace
-
-struct rlc_batch {
-  rlc_context *ctx = nullptr;
-  uint64_t vcap = 0, ecap = 0;
-  uint32_t maxg = 0;
-  bool weighted = false;
-  int cus = 256;
-  // arena
-  uint64_t *d_off = nullptr;
-  uint32_t *d_edg = nullptr;
-  float *d_wei = nullptr;
-  uint32_t *d_lab = nullptr;
-  uint8_t *d_aff = nullptr;
-  uint64_t *d_moves = nullptr;
-  // tables
-  BatchSlot *d_slots = nullptr;
-  uint32_t *d_active = nullptr;
-  uint64_t *d_prefix = nullptr;
-  BatchQueue *d_q = nullptr;
-  DevCounters *d_ctr = nullptr;
-  uint32_t *d_ids = nullptr;
-  uint64_t *d_vfirst = nullptr;
-  unsigned long long *d_bstats = nullptr;
-  uint32_t nbstats = 0;
-  // host bookkeeping
-  std::vector<HostSlot> slots;
-  std::vector<uint32_t> free_ids;  // stack, smallest id on top
-  RangeAlloc valloc, ealloc;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  hipGraph_t hg = nullptr;
-  hipGraphExec_t hx = nullptr;
-  BatchKey key;
-  rlc_batch_tuning tuning{};
-  std::atomic<int> busy{0};
-
-  BatchArgs args(const rlc_options &o, const rlc_batch_tuning &t) const {
-    BatchArgs a;
-    a.g = DevGraph{d_off, d_edg, weighted ? d_wei : nullptr, vcap, ecap};
-    a.s = DevState{d_lab, nullptr, d_aff, d_moves, d_ctr};
-    a.slots = d_slots;
-    a.nslots = maxg;
-    a.active = d_active;
-    a.prefix = d_prefix;
-    a.q = d_q;
-    a.pickstep = o.pickstep;
-    a.tolerance = o.tolerance;
-    a.max_iterations = o.max_iterations;
-    a.bstats = d_bstats;
-    a.nbstats = nbstats;
-    // asynchronous mode: kMaxSweeps sub-sweeps per step (options.sweeps = 1: a single sweep);
-    // the synchronous mode reads a snapshot, so one sweep
-    a.sweeps = (o.mode == RLC_MODE_SYNC || o.sweeps == 1) ? 1u : (uint32_t)kMaxSweeps;
-    a.interleave = (uint32_t)t.interleave;
-    return a;
-  }
-};
-
-namespace {
-
-void free_batch(rlc_batch *b) {
+void rlc::free_batch(rlc_batch *b) {
   if (!b) return;
   if (b->ctx) (void)hipSetDevice(b->ctx->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
@@ -132,7 +44,7 @@ void free_batch(rlc_batch *b) {
   delete b;
 }
 
-rlc_status check_tuning(const rlc_batch_tuning *t) {
+rlc_status rlc::check_tuning(const rlc_batch_tuning *t) {
   if (t->blocks_per_cu < 1 || t->blocks_per_cu > 16)
     return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.blocks_per_cu must be in 1..16");
   if (t->grab < 1 || t->grab > 256) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.grab must be in 1..256");
@@ -146,7 +58,7 @@ rlc_status check_tuning(const rlc_batch_tuning *t) {
   return RLC_OK;
 }
 
-rlc_status check_batch_options(const rlc_options *o) {
+rlc_status rlc::check_batch_options(const rlc_options *o) {
   rlc_status st = check_options(o);
   if (st != RLC_OK) return st;
   if (o->no_rescan) return fail(RLC_ERR_INVALID_ARGUMENT, "options.no_rescan is not supported by batches");
@@ -157,7 +69,7 @@ rlc_status check_batch_options(const rlc_options *o) {
 
 // Degree classes and grid of a run. SYNC: the classes are part of the result (merge tree of
 // options.merge_parts parts for degree >= merge_degree), exactly as rlc_run builds them.
-BatchTune make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t) {
+BatchTune rlc::make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t) {
   BatchTune bt;
   bt.blocks = (uint32_t)(b->cus * t.blocks_per_cu);
   bt.grab = (uint32_t)t.grab;
@@ -180,6 +92,52 @@ BatchTune make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tu
   }
   return bt;
 }
+
+rlc_status rlc::batch_place(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg,
+                            const float *wei, hipStream_t st, uint32_t *id) {
+  if (wei && !b->weighted) {
+    for (uint64_t i = 0; i < m; ++i)
+      if (wei[i] != 1.0f) return fail(RLC_ERR_INVALID_ARGUMENT, "batch: the arena stores no weights (weighted = 0)");
+  }
+  if (b->free_ids.empty()) return fail(RLC_ERR_CAPACITY, "batch: no free graph id");
+  uint64_t vb = 0, eb = 0;
+  if (!b->valloc.alloc(n + 1, vb)) return fail(RLC_ERR_CAPACITY, "batch: no free vertex range");
+  if (!b->ealloc.alloc(m, eb)) {
+    b->valloc.release(vb, n + 1);
+    return fail(RLC_ERR_CAPACITY, "batch: no free edge range");
+  }
+  const uint32_t g = b->free_ids.back();
+  auto undo = [&](rlc_status s) {
+    b->valloc.release(vb, n + 1);
+    b->ealloc.release(eb, m);
+    return s;
+  };
+  b->slot_stage[g] = BatchSlot{(uint32_t)vb, (uint32_t)n, 0u, (uint32_t)kSlotIdle, 0ull, 0ull};
+  if (hipMemcpyAsync(b->d_off + vb, off, (n + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+      (m && hipMemcpyAsync(b->d_edg + eb, edg, m * 4, hipMemcpyHostToDevice, st) != hipSuccess) ||
+      (m && b->weighted && wei && hipMemcpyAsync(b->d_wei + eb, wei, m * 4, hipMemcpyHostToDevice, st) != hipSuccess) ||
+      hipMemcpyAsync(b->d_slots + g, &b->slot_stage[g], sizeof(BatchSlot), hipMemcpyHostToDevice, st) != hipSuccess)
+    return undo(fail(RLC_ERR_DEVICE, "batch: graph copy failed"));
+  launch_batch_rebase(b->d_off + vb, n, eb, b->d_edg + eb, m, (uint32_t)vb, b->weighted && !wei ? b->d_wei + eb : nullptr,
+                      st);
+  rlc_status s2 = check_launch();
+  if (s2 != RLC_OK) return undo(s2);
+  b->free_ids.pop_back();
+  b->slots[g] = HostSlot{true, vb, n, eb, m};
+  *id = g;
+  return RLC_OK;
+}
+
+void rlc::batch_release(rlc_batch *b, uint32_t id) {
+  const HostSlot &h = b->slots[id];
+  b->valloc.release(h.vb, h.n + 1);
+  b->ealloc.release(h.eb, h.m);
+  b->slots[id] = HostSlot{};
+  b->free_ids.push_back(id);
+  std::sort(b->free_ids.begin(), b->free_ids.end(), std::greater<uint32_t>());
+}
+
+namespace {
 
 // Selected graph ids (all used ones when ids is null), validated, and their vertex prefix.
 rlc_status select_ids(const rlc_batch *b, const uint32_t *ids, uint32_t count, std::vector<uint32_t> &sel,
@@ -208,14 +166,6 @@ rlc_status upload_selection(rlc_batch *b, const std::vector<uint32_t> &sel, cons
   if (sel.empty()) return RLC_OK;
   HIP_TRY(hipMemcpyAsync(b->d_ids, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipMemcpyAsync(b->d_vfirst, vfirst.data(), vfirst.size() * 8, hipMemcpyHostToDevice, b->stream));
-  return RLC_OK;
-}
-
-rlc_status ensure_moves(rlc_batch *b, const rlc_options &o) {
-  if (o.mode != RLC_MODE_SYNC || b->d_moves) return RLC_OK;
-  rlc_status st = dev_alloc(&b->d_moves, b->vcap);
-  if (st != RLC_OK) return st;
-  b->key.valid = false;  // captured launches embed the state pointers
   return RLC_OK;
 }
 
@@ -301,6 +251,7 @@ rlc_status rlc_batch_create(rlc_context *ctx, uint64_t vertex_capacity, uint64_t
       hipStreamSynchronize(b->stream) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "batch initialisation failed"));
   b->slots.assign(max_graphs, HostSlot{});
+  b->slot_stage.assign(max_graphs, BatchSlot{});
   b->free_ids.resize(max_graphs);
   for (uint32_t i = 0; i < max_graphs; ++i) b->free_ids[i] = max_graphs - 1 - i;
   b->valloc.reset(vertex_capacity);
@@ -318,37 +269,13 @@ rlc_status rlc_batch_add(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *o
   uint64_t md = 0;
   rlc_status st = validate_csr(n, m, offsets, edges, weights, &md);
   if (st != RLC_OK) return st;
-  if (weights && !b->weighted) {
-    for (uint64_t i = 0; i < m; ++i)
-      if (weights[i] != 1.0f) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_batch_add: the batch stores no weights (weighted = 0)");
-  }
-  if (b->free_ids.empty()) return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free graph id");
-  uint64_t vb = 0, eb = 0;
-  if (!b->valloc.alloc(n + 1, vb)) return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free vertex range");
-  if (!b->ealloc.alloc(m, eb)) {
-    b->valloc.release(vb, n + 1);
-    return fail(RLC_ERR_CAPACITY, "rlc_batch_add: no free edge range");
-  }
-  const uint32_t g = b->free_ids.back();
   HIP_TRY(hipSetDevice(b->ctx->device));
-  auto undo = [&](rlc_status s) {
-    b->valloc.release(vb, n + 1);
-    b->ealloc.release(eb, m);
-    return s;
-  };
-  const BatchSlot slot{(uint32_t)vb, (uint32_t)n, 0u, (uint32_t)kSlotIdle, 0ull, 0ull};
-  if (hipMemcpyAsync(b->d_off + vb, offsets, (n + 1) * 8, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-      (m && hipMemcpyAsync(b->d_edg + eb, edges, m * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess) ||
-      (m && b->weighted && weights &&
-       hipMemcpyAsync(b->d_wei + eb, weights, m * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess) ||
-      hipMemcpyAsync(b->d_slots + g, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream) != hipSuccess)
-    return undo(fail(RLC_ERR_DEVICE, "rlc_batch_add: copy failed"));
-  launch_batch_rebase(b->d_off + vb, n, eb, b->d_edg + eb, m, (uint32_t)vb,
-                      b->weighted && !weights ? b->d_wei + eb : nullptr, b->stream);
-  if ((st = check_launch()) != RLC_OK) return undo(st);
-  if (hipStreamSynchronize(b->stream) != hipSuccess) return undo(fail(RLC_ERR_DEVICE, "rlc_batch_add: synchronise failed"));
-  b->free_ids.pop_back();
-  b->slots[g] = HostSlot{true, vb, n, eb, m};
+  uint32_t g = 0;
+  if ((st = batch_place(b, n, m, offsets, edges, weights, b->stream, &g)) != RLC_OK) return st;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) {
+    batch_release(b, g);
+    return fail(RLC_ERR_DEVICE, "rlc_batch_add: synchronise failed");
+  }
   *id = g;
   return RLC_OK;
 }
@@ -363,12 +290,7 @@ rlc_status rlc_batch_remove(rlc_batch *b, uint32_t id) {
   const BatchSlot slot{0u, 0u, 0u, (uint32_t)kSlotFree, 0ull, 0ull};
   HIP_TRY(hipMemcpyAsync(b->d_slots + id, &slot, sizeof(slot), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  const HostSlot &h = b->slots[id];
-  b->valloc.release(h.vb, h.n + 1);
-  b->ealloc.release(h.eb, h.m);
-  b->slots[id] = HostSlot{};
-  b->free_ids.push_back(id);
-  std::sort(b->free_ids.begin(), b->free_ids.end(), std::greater<uint32_t>());
+  batch_release(b, id);
   return RLC_OK;
 }
 

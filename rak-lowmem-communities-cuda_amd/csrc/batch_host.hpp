@@ -1,0 +1,129 @@
+// Host-side state of a batch arena, shared by the batch C ABI and the serving loop.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <vector>
+
+#include "arena.hpp"
+#include "batch.hpp"
+#include "context.hpp"
+
+namespace rlc {
+
+constexpr uint32_t kNever = 0xFFFFFFFFu;
+
+struct BatchKey {
+  bool valid = false;
+  int sync = 0, mg = 0, pickstep = 0, max_iterations = 0, sweeps = 0, interleave = 0;
+  double tolerance = 0;
+  BatchTune t{};
+  bool operator==(const BatchKey &o) const {
+    return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
+           max_iterations == o.max_iterations && sweeps == o.sweeps && interleave == o.interleave &&
+           tolerance == o.tolerance &&
+           t.blocks == o.t.blocks &&
+           t.grab == o.t.grab && t.group_degree == o.t.group_degree && t.group_lanes == o.t.group_lanes &&
+           t.wave_degree == o.t.wave_degree;
+  }
+};
+
+struct HostSlot {
+  bool used = false;
+  uint64_t vb = 0, n = 0, eb = 0, m = 0;
+};
+
+struct BusyFlag {
+  std::atomic<int> &b;
+  bool ok;
+  explicit BusyFlag(std::atomic<int> &x) : b(x) {
+    int z = 0;
+    ok = b.compare_exchange_strong(z, 1);
+  }
+  ~BusyFlag() {
+    if (ok) b.store(0);
+  }
+};
+
+}  // namespace rlc
+
+struct rlc_batch {
+  using BatchKey = rlc::BatchKey;
+  using HostSlot = rlc::HostSlot;
+  rlc_context *ctx = nullptr;
+  uint64_t vcap = 0, ecap = 0;
+  uint32_t maxg = 0;
+  bool weighted = false;
+  int cus = 256;
+  // arena
+  uint64_t *d_off = nullptr;
+  uint32_t *d_edg = nullptr;
+  float *d_wei = nullptr;
+  uint32_t *d_lab = nullptr;
+  uint8_t *d_aff = nullptr;
+  uint64_t *d_moves = nullptr;
+  // tables
+  rlc::BatchSlot *d_slots = nullptr;
+  uint32_t *d_active = nullptr;
+  uint64_t *d_prefix = nullptr;
+  rlc::BatchQueue *d_q = nullptr;
+  rlc::DevCounters *d_ctr = nullptr;
+  uint32_t *d_ids = nullptr;
+  uint64_t *d_vfirst = nullptr;
+  unsigned long long *d_bstats = nullptr;
+  uint32_t nbstats = 0;
+  // host bookkeeping
+  std::vector<HostSlot> slots;
+  std::vector<rlc::BatchSlot> slot_stage;  // host copy source of slot entries (stable while copies run)
+  std::vector<uint32_t> free_ids;  // stack, smallest id on top
+  rlc::RangeAlloc valloc, ealloc;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  hipGraph_t hg = nullptr;
+  hipGraphExec_t hx = nullptr;
+  BatchKey key;
+  rlc_batch_tuning tuning{};
+  std::atomic<int> busy{0};
+
+  rlc::BatchArgs args(const rlc_options &o, const rlc_batch_tuning &t) const {
+    rlc::BatchArgs a;
+    a.g = rlc::DevGraph{d_off, d_edg, weighted ? d_wei : nullptr, vcap, ecap};
+    a.s = rlc::DevState{d_lab, nullptr, d_aff, d_moves, d_ctr};
+    a.slots = d_slots;
+    a.nslots = maxg;
+    a.active = d_active;
+    a.prefix = d_prefix;
+    a.q = d_q;
+    a.pickstep = o.pickstep;
+    a.tolerance = o.tolerance;
+    a.max_iterations = o.max_iterations;
+    a.bstats = d_bstats;
+    a.nbstats = nbstats;
+    // asynchronous mode: kMaxSweeps sub-sweeps per step (options.sweeps = 1: a single sweep);
+    // the synchronous mode reads a snapshot, so one sweep
+    a.sweeps = (o.mode == RLC_MODE_SYNC || o.sweeps == 1) ? 1u : (uint32_t)rlc::kMaxSweeps;
+    a.interleave = (uint32_t)t.interleave;
+    return a;
+  }
+};
+
+
+namespace rlc {
+
+void free_batch(rlc_batch *b);
+rlc_status check_tuning(const rlc_batch_tuning *t);
+rlc_status check_batch_options(const rlc_options *o);
+// Degree classes and grid of a run (SYNC: the classes follow merge_degree / merge_parts).
+BatchTune make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t);
+// The change list of the synchronous mode (allocated on first use).
+rlc_status ensure_moves(rlc_batch *b, const rlc_options &o);
+// Takes a graph id and arena ranges for a validated CSR and enqueues its copy into the arena
+// (local values rebased to arena values) and its slot entry (idle) on stream st. The caller
+// synchronises st before the graph is used; on failure nothing is taken.
+rlc_status batch_place(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg,
+                       const float *wei, hipStream_t st, uint32_t *id);
+// Gives graph id's ranges and id back (host bookkeeping only).
+void batch_release(rlc_batch *b, uint32_t id);
+
+}  // namespace rlc

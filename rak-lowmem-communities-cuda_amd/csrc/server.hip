@@ -1,0 +1,426 @@
+// Serving loop: continuous batching of label-propagation requests on a resident arena.
+//
+// No counterpart in the reference (one graph per call, inc/rakLowmemCuda.hxx:538-607). One
+// worker thread per server owns an rlc_batch and repeats a round:
+//   1. graphs uploaded in the previous round are admitted on the compute stream (labels, flags,
+//      running state; after their copies, through an event), then the plan of the step and
+//      steps_per_round iterations of every resident graph (one captured hipGraph) are launched;
+//   2. while those run, queued requests are admitted in arrival order while they fit (a free
+//      graph id and free vertex / edge ranges of the arena) and their CSR copied on the copy
+//      stream;
+//   3. after the round, the slot table is read back; finished graphs have their labels copied to
+//      the host and their space freed.
+// Each graph keeps its own iteration count and convergence test in its slot, so a graph's run
+// is the same in any mix (bit for bit in the synchronous mode).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.hpp"
+#include "batch_host.hpp"
+#include "context.hpp"
+#include "host.hpp"
+
+using namespace rlc;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+enum class ReqState { Queued, Uploaded, Running, Done, Failed };
+
+struct Request {
+  uint64_t id = 0;
+  uint64_t n = 0, m = 0;
+  const uint64_t *off = nullptr;
+  const uint32_t *edg = nullptr;
+  const float *wei = nullptr;
+  ReqState state = ReqState::Queued;
+  rlc_status status = RLC_OK;
+  std::string error;
+  uint32_t gid = 0;
+  int32_t iterations = 0;
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  std::vector<uint32_t> labels;
+};
+
+}  // namespace
+
+struct rlc_server {
+  rlc_context *ctx = nullptr;
+  rlc_batch *b = nullptr;
+  rlc_options opts{};
+  rlc_batch_tuning tune{};
+  int steps = 2;
+  Clock::time_point t0;
+  hipStream_t compute = nullptr, copy = nullptr;
+  hipEvent_t ev_copy = nullptr, ev_round = nullptr;
+  hipGraph_t hg = nullptr;
+  hipGraphExec_t hx = nullptr;
+  // requests
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> queue;                    // queued ids, arrival order
+  std::unordered_map<uint64_t, Request> reqs;    // every request not yet collected
+  uint64_t next_id = 1;
+  uint64_t submitted = 0, finished = 0, failed = 0;
+  // worker
+  std::thread worker;
+  std::atomic<bool> stop{false};
+  std::vector<uint64_t> resident;  // running on the device (request ids)
+  std::vector<uint64_t> uploaded;  // copied this round, admitted next round
+  uint64_t rounds = 0, steps_run = 0;
+  uint32_t resident_max = 0;
+  std::string worker_error;
+
+  double now() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+// Marks request r failed (under the lock) and wakes waiters.
+void fail_request(rlc_server *S, Request &r, rlc_status st, const std::string &msg) {
+  r.state = ReqState::Failed;
+  r.status = st;
+  r.error = msg;
+  r.t_done = S->now();
+  ++S->finished;
+  ++S->failed;
+  S->cv_done.notify_all();
+}
+
+// Step 2 of a round: copy queued requests into the arena while they fit (arrival order).
+void admit_queued(rlc_server *S) {
+  rlc_batch *b = S->b;
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      if (S->queue.empty()) break;
+      r = &S->reqs[S->queue.front()];  // only the worker erases queued requests
+    }
+    uint32_t gid = 0;
+    const double t = S->now();
+    const rlc_status st = batch_place(b, r->n, r->m, r->off, r->edg, r->wei, S->copy, &gid);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (st == RLC_ERR_CAPACITY) break;  // wait for space: arrival order is kept
+    S->queue.pop_front();
+    if (st != RLC_OK) {
+      fail_request(S, *r, st, last_error_cstr());
+      continue;
+    }
+    r->gid = gid;
+    r->t_admit = t;
+    r->state = ReqState::Uploaded;
+    S->uploaded.push_back(r->id);
+  }
+}
+
+// Step 1: admission kernels for the graphs uploaded last round, then the round's iterations.
+rlc_status launch_round(rlc_server *S, std::vector<uint64_t> &fresh) {
+  rlc_batch *b = S->b;
+  const BatchArgs a = b->args(S->opts, S->tune);
+  if (!fresh.empty()) {
+    std::vector<uint32_t> ids;
+    std::vector<uint64_t> vfirst(1, 0);
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      for (uint64_t rid : fresh) {
+        Request &r = S->reqs[rid];
+        ids.push_back(r.gid);
+        vfirst.push_back(vfirst.back() + r.n);
+        r.state = ReqState::Running;
+        S->resident.push_back(rid);
+      }
+    }
+    HIP_TRY(hipStreamWaitEvent(S->compute, S->ev_copy, 0));
+    HIP_TRY(hipMemcpyAsync(b->d_ids, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, S->compute));
+    HIP_TRY(hipMemcpyAsync(b->d_vfirst, vfirst.data(), vfirst.size() * 8, hipMemcpyHostToDevice, S->compute));
+    launch_batch_admit(a, b->d_ids, b->d_vfirst, (uint32_t)ids.size(), vfirst.back(), nullptr, false, S->compute);
+    HIP_TRY(hipStreamSynchronize(S->compute));  // ids / vfirst are read by the admission kernel
+    S->resident_max = std::max<uint32_t>(S->resident_max, (uint32_t)S->resident.size());
+  }
+  launch_batch_plan(a, S->compute);
+  HIP_TRY(hipGraphLaunch(S->hx, S->compute));
+  HIP_TRY(hipEventRecord(S->ev_round, S->compute));
+  ++S->rounds;
+  S->steps_run += (uint64_t)S->steps;
+  return check_launch();
+}
+
+// Step 3: collect the graphs that finished.
+rlc_status retire(rlc_server *S) {
+  rlc_batch *b = S->b;
+  std::vector<BatchSlot> slots(b->maxg);
+  HIP_TRY(hipMemcpyAsync(slots.data(), b->d_slots, slots.size() * sizeof(BatchSlot), hipMemcpyDeviceToHost,
+                         S->compute));
+  HIP_TRY(hipStreamSynchronize(S->compute));
+  std::vector<uint64_t> keep, done;
+  for (uint64_t rid : S->resident) {
+    Request *r;
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      r = &S->reqs[rid];
+    }
+    if (slots[r->gid].state != kSlotDone) {
+      keep.push_back(rid);
+      continue;
+    }
+    r->labels.resize(r->n);
+    if (r->n)
+      HIP_TRY(hipMemcpyAsync(r->labels.data(), b->d_lab + b->slots[r->gid].vb, r->n * 4, hipMemcpyDeviceToHost,
+                             S->compute));
+    r->iterations = (int32_t)slots[r->gid].iter;
+    done.push_back(rid);
+  }
+  if (done.empty()) return RLC_OK;
+  HIP_TRY(hipStreamSynchronize(S->compute));
+  const double t = S->now();
+  std::lock_guard<std::mutex> lk(S->mu);
+  for (uint64_t rid : done) {
+    Request &r = S->reqs[rid];
+    const uint32_t vb = (uint32_t)b->slots[r.gid].vb;
+    for (auto &x : r.labels) x -= vb;
+    batch_release(b, r.gid);
+    r.state = ReqState::Done;
+    r.t_done = t;
+    ++S->finished;
+  }
+  S->resident.swap(keep);
+  S->cv_done.notify_all();
+  return RLC_OK;
+}
+
+void worker_main(rlc_server *S) {
+  if (hipSetDevice(S->ctx->device) != hipSuccess) {
+    S->worker_error = "hipSetDevice failed";
+    return;
+  }
+  for (;;) {
+    std::vector<uint64_t> fresh;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->uploaded.empty() || !S->resident.empty(); });
+      if (S->stop) return;
+      fresh.swap(S->uploaded);
+    }
+    const bool run = !fresh.empty() || !S->resident.empty();
+    rlc_status st = RLC_OK;
+    if (run) st = launch_round(S, fresh);
+    if (st == RLC_OK) {
+      admit_queued(S);  // overlaps the round on the device
+      if (!S->uploaded.empty() && hipEventRecord(S->ev_copy, S->copy) != hipSuccess) st = RLC_ERR_DEVICE;
+    }
+    if (st == RLC_OK && run) {
+      if (hipEventSynchronize(S->ev_round) != hipSuccess) st = fail(RLC_ERR_DEVICE, "round failed on the device");
+      else st = retire(S);
+    }
+    if (st != RLC_OK) {
+      // a device error ends the server: fail everything outstanding
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->worker_error = last_error_cstr();
+      for (auto &kv : S->reqs)
+        if (kv.second.state != ReqState::Done && kv.second.state != ReqState::Failed)
+          fail_request(S, kv.second, st, S->worker_error);
+      S->queue.clear();
+      S->resident.clear();
+      S->uploaded.clear();
+      return;
+    }
+  }
+}
+
+void free_server(rlc_server *S) {
+  if (!S) return;
+  if (S->worker.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(S->mu);
+      S->stop = true;
+    }
+    S->cv_work.notify_all();
+    S->worker.join();
+  }
+  if (S->ctx) (void)hipSetDevice(S->ctx->device);
+  if (S->compute) (void)hipStreamSynchronize(S->compute);
+  if (S->copy) (void)hipStreamSynchronize(S->copy);
+  if (S->hx) (void)hipGraphExecDestroy(S->hx);
+  if (S->hg) (void)hipGraphDestroy(S->hg);
+  if (S->ev_copy) (void)hipEventDestroy(S->ev_copy);
+  if (S->ev_round) (void)hipEventDestroy(S->ev_round);
+  if (S->compute) (void)hipStreamDestroy(S->compute);
+  if (S->copy) (void)hipStreamDestroy(S->copy);
+  if (S->b) free_batch(S->b);
+  delete S;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rlc_server_config_default(rlc_server_config *c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof(*c));
+  c->vertex_capacity = 1ull << 26;
+  c->edge_capacity = 1ull << 30;
+  c->max_graphs = 4096;
+  c->steps_per_round = 2;
+}
+
+rlc_status rlc_server_create(rlc_context *ctx, const rlc_server_config *c, const rlc_options *o,
+                             const rlc_batch_tuning *t, rlc_server **out) {
+  clear_error();
+  if (!ctx || !c || !out) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_create: null argument");
+  *out = nullptr;
+  rlc_status st = check_batch_options(o);
+  if (st != RLC_OK) return st;
+  if (t && (st = check_tuning(t)) != RLC_OK) return st;
+  if (c->steps_per_round < 1 || c->steps_per_round > 64)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "config.steps_per_round must be in 1..64");
+  for (int r : c->reserved)
+    if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "config.reserved must be zero");
+  if (o->max_iterations < 1) return fail(RLC_ERR_INVALID_ARGUMENT, "options.max_iterations must be >= 1 for a server");
+  rlc_batch *b = nullptr;
+  if ((st = rlc_batch_create(ctx, c->vertex_capacity, c->edge_capacity, c->max_graphs, c->weighted, &b)) != RLC_OK)
+    return st;
+  rlc_server *S = new rlc_server();
+  S->ctx = ctx;
+  S->b = b;
+  S->opts = *o;
+  S->steps = c->steps_per_round;
+  if (t) S->tune = *t;
+  else rlc_batch_tuning_default(&S->tune);
+  S->t0 = Clock::now();
+  auto bail = [&](rlc_status s) {
+    free_server(S);
+    return s;
+  };
+  if (hipStreamCreateWithFlags(&S->compute, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&S->ev_copy, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&S->ev_round, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: stream or event creation failed"));
+  if ((st = ensure_moves(b, *o)) != RLC_OK) return bail(st);
+  // the round: steps_per_round steps of every running graph, captured once
+  const BatchTune bt = make_tune(b, *o, S->tune);
+  const BatchArgs a = b->args(*o, S->tune);
+  const BatchParams p{o->mode == RLC_MODE_SYNC, o->slots == RLC_SKETCH_MG8};
+  if (hipStreamBeginCapture(S->compute, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: capture failed"));
+  for (int l = 0; l < S->steps; ++l) launch_batch_step(a, bt, p, S->compute);
+  hipGraph_t hg = nullptr;
+  if (hipStreamEndCapture(S->compute, &hg) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: capture failed"));
+  S->hg = hg;
+  if (hipGraphInstantiate(&S->hx, S->hg, nullptr, nullptr, 0) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: graph instantiation failed"));
+  S->worker = std::thread(worker_main, S);
+  *out = S;
+  return RLC_OK;
+}
+
+rlc_status rlc_server_submit(rlc_server *S, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
+                             const float *weights, uint64_t *request) {
+  clear_error();
+  if (!S || !offsets || !request) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_submit: null argument");
+  uint64_t md = 0;
+  rlc_status st = validate_csr(n, m, offsets, edges, weights, &md);
+  if (st != RLC_OK) return st;
+  if (weights && !S->b->weighted) {
+    for (uint64_t i = 0; i < m; ++i)
+      if (weights[i] != 1.0f) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_submit: the server stores no weights");
+  }
+  if (n + 1 > S->b->vcap || m > S->b->ecap)
+    return fail(RLC_ERR_CAPACITY, "rlc_server_submit: the graph can never fit the server's arena");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (!S->worker_error.empty()) return fail(RLC_ERR_DEVICE, "rlc_server_submit: server stopped: " + S->worker_error);
+  Request r;
+  r.id = S->next_id++;
+  r.n = n;
+  r.m = m;
+  r.off = offsets;
+  r.edg = edges;
+  r.wei = weights;
+  r.t_submit = S->now();
+  const uint64_t id = r.id;
+  S->reqs.emplace(id, std::move(r));
+  S->queue.push_back(id);
+  ++S->submitted;
+  S->cv_work.notify_all();
+  *request = id;
+  return RLC_OK;
+}
+
+rlc_status rlc_server_result(rlc_server *S, uint64_t request, uint32_t *labels, rlc_request_info *info) {
+  clear_error();
+  if (!S) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_result: null server");
+  std::lock_guard<std::mutex> lk(S->mu);
+  auto it = S->reqs.find(request);
+  if (it == S->reqs.end()) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_result: unknown request");
+  Request &r = it->second;
+  if (r.state != ReqState::Done && r.state != ReqState::Failed)
+    return fail(RLC_ERR_NOT_READY, "rlc_server_result: request not finished");
+  if (info) {
+    info->status = r.status;
+    info->iterations = r.iterations;
+    info->n = r.n;
+    info->m = r.m;
+    info->submit_ms = r.t_submit;
+    info->admit_ms = r.t_admit;
+    info->done_ms = r.t_done;
+  }
+  const rlc_status st = r.status;
+  const std::string err = r.error;
+  if (st == RLC_OK && labels && r.n) std::memcpy(labels, r.labels.data(), r.n * 4);
+  S->reqs.erase(it);
+  if (st != RLC_OK) return fail(st, "rlc_server_result: request failed: " + err);
+  return RLC_OK;
+}
+
+rlc_status rlc_server_drain(rlc_server *S, int64_t timeout_ms) {
+  clear_error();
+  if (!S) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_drain: null server");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto all_done = [&] { return S->finished == S->submitted; };
+  if (timeout_ms < 0) S->cv_done.wait(lk, all_done);
+  else if (!S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), all_done))
+    return fail(RLC_ERR_NOT_READY, "rlc_server_drain: timeout");
+  if (!S->worker_error.empty()) return fail(RLC_ERR_DEVICE, "rlc_server_drain: server stopped: " + S->worker_error);
+  return RLC_OK;
+}
+
+rlc_status rlc_server_stats_get(rlc_server *S, rlc_server_stats *st) {
+  clear_error();
+  if (!S || !st) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_server_stats_get: null argument");
+  std::memset(st, 0, sizeof(*st));
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    st->submitted = S->submitted;
+    st->completed = S->finished - S->failed;
+    st->failed = S->failed;
+    st->rounds = S->rounds;
+    st->steps = S->steps_run;
+    st->resident_max = S->resident_max;
+  }
+  // traversed edges: the move kernels' per-workgroup counters (read with a blocking copy on the
+  // null stream of this thread, which does not wait for the worker's streams)
+  std::vector<unsigned long long> bs(4ull * S->b->nbstats);
+  HIP_TRY(hipSetDevice(S->ctx->device));
+  HIP_TRY(hipMemcpy(bs.data(), S->b->d_bstats, bs.size() * 8, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < S->b->nbstats; ++i) st->traversed_edges += bs[4ull * i + 1];
+  return RLC_OK;
+}
+
+double rlc_server_now_ms(const rlc_server *S) { return S ? S->now() : 0.0; }
+
+void rlc_server_destroy(rlc_server *S) { free_server(S); }
+
+}  // extern "C"

@@ -84,6 +84,22 @@ class CsrSet(C.Structure):
                 ("weights", C.POINTER(C.c_float))]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("vertex_capacity", C.c_uint64), ("edge_capacity", C.c_uint64), ("max_graphs", C.c_uint32),
+                ("weighted", C.c_int32), ("steps_per_round", C.c_int32), ("reserved", C.c_int32 * 3)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("n", C.c_uint64), ("m", C.c_uint64),
+                ("submit_ms", C.c_double), ("admit_ms", C.c_double), ("done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("rounds", C.c_uint64), ("steps", C.c_uint64), ("traversed_edges", C.c_uint64),
+                ("resident_max", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 # (name, restype, argtypes) of every exported entry point, in include/rlc.h order
 _P = C.c_void_p
 SIGNATURES = [
@@ -112,6 +128,15 @@ SIGNATURES = [
     ("rlc_batch_result", C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_int32)]),
     ("rlc_batch_tune", C.c_int, [_P, _P, C.c_uint32, C.POINTER(Options), C.POINTER(BatchTuning)]),
     ("rlc_batch_destroy", None, [_P]),
+    ("rlc_server_config_default", None, [C.POINTER(ServerConfig)]),
+    ("rlc_server_create", C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(Options), C.POINTER(BatchTuning),
+                                    C.POINTER(_P)]),
+    ("rlc_server_submit", C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, C.POINTER(C.c_uint64)]),
+    ("rlc_server_result", C.c_int, [_P, C.c_uint64, _P, C.POINTER(RequestInfo)]),
+    ("rlc_server_drain", C.c_int, [_P, C.c_int64]),
+    ("rlc_server_stats_get", C.c_int, [_P, C.POINTER(ServerStats)]),
+    ("rlc_server_now_ms", C.c_double, [_P]),
+    ("rlc_server_destroy", None, [_P]),
     ("rlc_gen_sbm", C.c_int, [C.c_uint64, C.c_uint32, C.c_double, C.c_double, C.c_uint64, C.POINTER(Csr)]),
     ("rlc_gen_rmat", C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_float,
                                C.c_float, C.c_uint64, C.POINTER(Csr)]),
@@ -388,6 +413,79 @@ class Batch:
         t = BatchTuning()
         _check(lib().rlc_batch_tune(self.h, _ptr(a), k, C.byref(o), C.byref(t)))
         return t
+
+
+def server_config(**kw) -> ServerConfig:
+    c = ServerConfig()
+    lib().rlc_server_config_default(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching request loop (rlc_server_*): a native worker thread admits requests
+    into a resident arena by footprint and advances every resident graph together."""
+
+    def __init__(self, ctx: Context, config: ServerConfig, opts: Options | None = None,
+                 tune: BatchTuning | None = None):
+        h = C.c_void_p()
+        o = opts if opts is not None else options()
+        _check(lib().rlc_server_create(ctx.h, C.byref(config), C.byref(o), None if tune is None else C.byref(tune),
+                                       C.byref(h)))
+        self.h = h
+        self.pending = {}  # request -> the arrays it reads (kept alive until collected)
+
+    def close(self):
+        if self.h:
+            lib().rlc_server_destroy(self.h)
+            self.h = None
+            self.pending.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, g: Graph) -> int:
+        off = np.ascontiguousarray(g.offsets, dtype=np.uint64)
+        edg = np.ascontiguousarray(g.edges, dtype=np.uint32)
+        wei = None if g.weights is None else np.ascontiguousarray(g.weights, dtype=np.float32)
+        rid = C.c_uint64()
+        _check(lib().rlc_server_submit(self.h, g.n, g.m, _ptr(off), _ptr(edg), _ptr(wei), C.byref(rid)))
+        self.pending[rid.value] = (off, edg, wei, g.n)
+        return rid.value
+
+    def result(self, rid: int, wait: bool = False):
+        """(labels, RequestInfo) of a finished request, or None while it runs (wait=False)."""
+        n = self.pending[rid][3]
+        lab = np.empty(n, np.uint32)
+        info = RequestInfo()
+        while True:
+            st = lib().rlc_server_result(self.h, rid, _ptr(lab), C.byref(info))
+            if st == 8 and wait:  # RLC_ERR_NOT_READY
+                import time
+                time.sleep(0.0005)
+                continue
+            if st == 8:
+                return None
+            self.pending.pop(rid, None)
+            _check(st)
+            return lab, info
+
+    def drain(self, timeout_ms: int = -1) -> None:
+        _check(lib().rlc_server_drain(self.h, timeout_ms))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().rlc_server_stats_get(self.h, C.byref(st)))
+        return st
+
+    def now_ms(self) -> float:
+        return float(lib().rlc_server_now_ms(self.h))
 
 
 class DeviceGraph:

@@ -234,8 +234,8 @@ def test_batch_errors(ctx):
 @pytest.mark.parametrize("slots", [8, 1])
 def test_batch_async_quality(ctx, slots):
     # Asynchronous batches update labels in place, 8 sub-sweeps per iteration. Quality gate: the
-    # modularity summed over the mix within 1% of the reference CPU path's (restated in the
-    # oracle) and of rlc_run's asynchronous result on each graph alone; per graph, no worse than
+    # modularity summed over the mix within 1% (MG8; 2% for Boyer-Moore) of the reference CPU
+    # path's (restated in the oracle) and of rlc_run's asynchronous result on each graph alone; per graph, no worse than
     # rlc_run alone by more than 0.05. (On the tiniest blocks, sequential Gauss-Seidel updates of
     # the reference can beat any parallel order on single graphs; the sum is the stable measure.)
     graphs = small_mix(12, seed=21, n_max=40000)
@@ -255,6 +255,8 @@ def test_batch_async_quality(ctx, slots):
         qs, qr, q1 = np.array(qs), np.array(qr), np.array(q1)
         print("async batch modularity:", np.round(qs, 4), "\nreference:", np.round(qr, 4),
               "\nrlc_run:", np.round(q1, 4), flush=True)
-        assert qs.sum() >= 0.99 * qr.sum()
-        assert qs.sum() >= 0.99 * q1.sum()
+        # MG8 within 1% of the reference; the one-slot Boyer-Moore sketch within 2%
+        tol = 0.01 if slots == 8 else 0.02
+        assert qs.sum() >= (1 - tol) * qr.sum()
+        assert qs.sum() >= (1 - tol) * q1.sum()
         assert (qs >= q1 - 0.05).all()

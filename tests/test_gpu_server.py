@@ -1,0 +1,154 @@
+"""Serving loop on the device: requests admitted into a resident arena as space frees up.
+
+A graph's run keeps its own iteration count and convergence test, so in the synchronous mode its
+labels are those of rlc_run on the graph alone, whatever else is resident (bit for bit). In the
+asynchronous mode (the serving default) each result is held to a tolerance instead: its FP64
+modularity in the mix within 0.02 of the same graph served alone.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rlc
+from graphs import cliques, from_pairs, star, with_hubs
+
+pytestmark = pytest.mark.gpu
+
+ASYNC_TOL = 0.02  # |Q(in the mix) - Q(alone)|, FP64 modularity
+
+
+def mix(count, seed, n_max=20000):
+    p = rlc.sbm_mix_params(count, seed, n_min=200, n_max=n_max)
+    s = rlc.gen_sbm_set(*p)
+    return [s.graph(i) for i in range(len(s))]
+
+
+def capacity_for(graphs, slack=1.0):
+    return int(sum(g.n + 1 for g in graphs) * slack) + 1, int(sum(g.m for g in graphs) * slack) + 1
+
+
+def serve_all(ctx, graphs, opts, cfg):
+    with rlc.Server(ctx, cfg, opts) as srv:
+        ids = [srv.submit(g) for g in graphs]
+        srv.drain(60000)
+        out = [srv.result(i) for i in ids]
+        st = srv.stats()
+    return out, st
+
+
+def single(ctx, g, opts):
+    with ctx.upload(g) as dg:
+        r = dg.run(opts)
+    return r.membership, r.iterations
+
+
+@pytest.mark.parametrize("slots", [8, 1])
+def test_server_sync_mix_equals_alone(ctx, slots):
+    graphs = mix(20, 3) + [star(200), cliques([6, 10, 3]), from_pairs(5, [(0, 1), (3, 4)])]
+    opts = rlc.options(mode=rlc.MODE_SYNC, slots=slots)
+    V, E = capacity_for(graphs)
+    out, st = serve_all(ctx, graphs, opts, rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=64))
+    print(f"rounds={st.rounds} steps={st.steps} resident_max={st.resident_max} completed={st.completed}", flush=True)
+    assert st.completed == len(graphs) and st.failed == 0
+    for g, (lab, info) in zip(graphs, out):
+        ref, it = single(ctx, g, opts)
+        assert info.iterations == it and np.array_equal(lab, ref)
+        assert info.submit_ms <= info.admit_ms <= info.done_ms
+
+
+def test_server_admission_by_footprint(ctx):
+    # an arena that holds about a quarter of the requests at once: they queue and are admitted as
+    # space frees up, in arrival order, with the same results
+    graphs = mix(16, 9, n_max=6000)
+    opts = rlc.options(mode=rlc.MODE_SYNC)
+    V, E = capacity_for(graphs, 0.25)
+    V = max(V, max(g.n + 1 for g in graphs))
+    E = max(E, max(g.m for g in graphs))
+    cfg = rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=64, steps_per_round=1)
+    out, st = serve_all(ctx, graphs, opts, cfg)
+    assert st.completed == len(graphs)
+    assert st.resident_max < len(graphs)
+    admits = [info.admit_ms for _, info in out]
+    assert admits == sorted(admits)  # arrival order
+    for g, (lab, info) in zip(graphs, out):
+        ref, it = single(ctx, g, opts)
+        assert info.iterations == it and np.array_equal(lab, ref)
+
+
+def test_server_slot_limit(ctx):
+    graphs = mix(12, 10, n_max=3000)
+    opts = rlc.options(mode=rlc.MODE_SYNC)
+    V, E = capacity_for(graphs)
+    out, st = serve_all(ctx, graphs, opts, rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=3))
+    assert st.completed == len(graphs) and st.resident_max <= 3
+    for g, (lab, info) in zip(graphs, out):
+        ref, it = single(ctx, g, opts)
+        assert info.iterations == it and np.array_equal(lab, ref)
+
+
+def test_server_continuous_arrivals(ctx):
+    # a large graph first, small ones arriving while it runs: the small ones are admitted into the
+    # running mix and finish before it
+    big = with_hubs(60000, hubs=20, hub_degree=3000, base_degree=24, seed=4, weighted=False)
+    small = mix(8, 12, n_max=2000)
+    opts = rlc.options(mode=rlc.MODE_SYNC, max_iterations=20)
+    V, E = capacity_for([big] + small)
+    with rlc.Server(ctx, rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=16, steps_per_round=1),
+                    opts) as srv:
+        rb = srv.submit(big)
+        time.sleep(0.002)
+        rs = [srv.submit(g) for g in small]
+        srv.drain(60000)
+        lb, ib = srv.result(rb)
+        res = [srv.result(r) for r in rs]
+    ref, it = single(ctx, big, opts)
+    assert ib.iterations == it and np.array_equal(lb, ref)
+    for g, (lab, info) in zip(small, res):
+        ref, it = single(ctx, g, opts)
+        assert info.iterations == it and np.array_equal(lab, ref)
+    print("big: admit %.2f done %.2f; small done:" % (ib.admit_ms, ib.done_ms), [round(i.done_ms, 2) for _, i in res])
+    if ib.iterations > 4:
+        assert sum(i.admit_ms < ib.done_ms for _, i in res) >= 1  # admitted while the big one ran
+
+
+def test_server_async_mix_within_tolerance(ctx):
+    graphs = mix(16, 5, n_max=30000)
+    opts = rlc.options()  # MG8, asynchronous
+    V, E = capacity_for(graphs)
+    cfg = rlc.server_config(vertex_capacity=V, edge_capacity=E, max_graphs=32)
+    out, st = serve_all(ctx, graphs, opts, cfg)
+    assert st.completed == len(graphs)
+    with rlc.Server(ctx, cfg, opts) as srv:  # each graph served alone
+        alone = []
+        for g in graphs:
+            rid = srv.submit(g)
+            alone.append(srv.result(rid, wait=True))
+    d = []
+    for g, (lab, info), (lab1, info1) in zip(graphs, out, alone):
+        assert lab.max() < g.n
+        q_mix, q_alone = O.modularity(g, lab), O.modularity(g, lab1)
+        d.append(q_mix - q_alone)
+    print("async Q(mix) - Q(alone):", np.round(d, 4), flush=True)
+    assert np.abs(d).max() <= ASYNC_TOL
+
+
+def test_server_errors(ctx):
+    cfg = rlc.server_config(vertex_capacity=100, edge_capacity=1000, max_graphs=4)
+    with rlc.Server(ctx, cfg, rlc.options(mode=rlc.MODE_SYNC)) as srv:
+        with pytest.raises(rlc.RlcError) as e:
+            srv.submit(star(200))  # can never fit
+        assert e.value.status == 6
+        with pytest.raises(rlc.RlcError):
+            srv.submit(rlc.Graph(np.array([0, 1], np.uint64), np.array([3], np.uint32), None))
+        rid = srv.submit(star(5))
+        lab, info = srv.result(rid, wait=True)
+        assert info.status == 0 and info.iterations >= 1
+        with pytest.raises(rlc.RlcError):
+            lib_result_again = rlc.lib().rlc_server_result(srv.h, rid, None, None)
+            rlc._check(lib_result_again)
+    with pytest.raises(rlc.RlcError):
+        rlc.Server(ctx, rlc.server_config(steps_per_round=0))
+    with pytest.raises(rlc.RlcError):
+        rlc.Server(ctx, cfg, rlc.options(max_iterations=0))
