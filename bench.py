@@ -53,7 +53,8 @@ def copy_probe(nbytes):
 def load_traffic(args):
     """HBM bytes per move-phase iteration measured by rocprofv3 counters on this workload (profiles/)."""
     path = os.path.join(ROOT, "profiles", "rmat24_mg8_async_traffic.json")
-    if args.scale != 24 or args.slots != 8 or args.group_degree != "0,0,0,0" or args.sweeps != 0:
+    if (args.workload != "rmat" or args.scale != 24 or args.unit or args.slots != 8 or args.group_degree != "0,0,0,0"
+            or args.sweeps != 0):
         return None
     try:
         with open(path) as f:
@@ -297,7 +298,9 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
-    ap.add_argument("--workload", choices=["rmat", "batch", "serve"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "grid", "batch", "serve"], default="rmat")
+    ap.add_argument("--unit", action="store_true", help="rmat workload: unit weights (the scale-27 configuration)")
+    ap.add_argument("--grid-side", type=int, default=10000, help="grid workload: lattice side (10000: 1e8 vertices)")
     ap.add_argument("--requests", type=int, default=1000, help="serve workload: requests")
     ap.add_argument("--load", type=float, default=0.8, help="serve workload: arrival rate / burst capacity")
     ap.add_argument("--serve-gb", type=float, default=24.0, help="serve workload: arena size (GiB)")
@@ -327,8 +330,17 @@ def main():
     import rlc
 
     t0 = time.time()
-    g = rlc.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, 0.5, 1.5, seed=1)
-    log(f"graph: R-MAT scale {args.scale}: n={g.n} m={g.m} max_degree={int(g.degrees().max())} "
+    if args.workload == "grid":
+        side = args.grid_side
+        g = rlc.gen_grid(side, side, 0.10, 0.02, seed=1)
+        name = (f"road-like grid {side}x{side} (4-neighbour lattice, 10% of lattice edges dropped, shortcuts "
+                f"for 2% of the kept edges), unit weights")
+    else:
+        wmin, wmax = (1.0, 1.0) if args.unit else (0.5, 1.5)
+        g = rlc.gen_rmat(args.scale, 16, 0.57, 0.19, 0.19, wmin, wmax, seed=1)
+        name = (f"R-MAT scale {args.scale}, edge factor 16, a=.57 b=.19 c=.19, symmetrised+deduplicated, "
+                f"{'unit weights' if args.unit else 'w~U[0.5,1.5)'}")
+    log(f"graph: {name}: n={g.n} m={g.m} average degree {g.m / g.n:.2f} max_degree={int(g.degrees().max())} "
         f"({time.time() - t0:.1f}s to generate)")
     opts = rlc.options(mode=rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
                        group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps)
@@ -373,8 +385,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"R-MAT scale {args.scale}, edge factor 16, a=.57 b=.19 c=.19, symmetrised+deduplicated, "
-                        f"w~U[0.5,1.5), {'MG8' if args.slots == 8 else 'BM'} async, tol 0.05, max 20 iterations",
+            "workload": f"{name}, {'MG8' if args.slots == 8 else 'BM'} async, tol 0.05, max 20 iterations",
             "n": g.n, "m": g.m, "iterations": iters, "modularity": q, "communities": ncom,
         },
         "roofline": {
@@ -397,11 +408,16 @@ def main():
 
         cores = len(os.sched_getaffinity(0))
         cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
-        lab, it, tms, trav = oracle.lowmem_omp(g, True, threads=cores)
+        # a bounded sample: the whole run when the graph is small enough, else its first iteration
+        # (every vertex active: the iteration with the most traversed edges)
+        full = g.m <= (1 << 30)
+        lab, it, tms, trav = oracle.lowmem_omp(g, args.slots == 8, threads=cores,
+                                               max_iterations=20 if full else 1)
+        what = "one full run" if full else "the first iteration (all vertices active)"
         line["cpu_baseline"] = {
             "value": trav / (tms * 1e-3), "unit": "edges/s", "cores": cores, "kind": "port",
-            "sample": f"one full MG8 run (rakLowmemStaticOmp<true,true,8> restated) on the same R-MAT scale "
-                      f"{args.scale} graph: {it} iterations, {trav} traversed edges, {tms / 1e3:.2f} s",
+            "sample": f"{what} of {'MG8' if args.slots == 8 else 'BM'} (rakLowmemStaticOmp restated) on the same "
+                      f"graph: {it} iterations, {trav} traversed edges, {tms / 1e3:.2f} s",
         }
     print(json.dumps(line), flush=True)
 

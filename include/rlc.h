@@ -304,15 +304,17 @@ typedef struct rlc_csr {
   uint64_t n, m;
   uint64_t *offsets; /* n+1 */
   uint32_t *edges;   /* m */
-  float *weights;    /* m */
+  float *weights;    /* m, or NULL when every weight is 1 */
 } rlc_csr;
 
 /* Stochastic block model: n vertices in `blocks` near-equal blocks, expected degree `degree`,
- * fraction `intra` of edges inside blocks; undirected, no self-loops, sorted adjacency, unit weights. */
+ * fraction `intra` of edges inside blocks; undirected, no self-loops, sorted adjacency, unit
+ * weights (NULL weights). */
 rlc_status rlc_gen_sbm(uint64_t n, uint32_t blocks, double degree, double intra, uint64_t seed, rlc_csr *out);
 /* R-MAT: 2^scale vertices, edge_factor * 2^scale generated edges with quadrant probabilities
  * a, b, c (d = 1-a-b-c); symmetrised, deduplicated, self-loops dropped, sorted adjacency.
- * wmin == wmax gives constant weights, else w ~ U[wmin, wmax) hashed per unordered pair. */
+ * wmin == wmax gives constant weights (NULL weights when 1), else w ~ U[wmin, wmax) hashed per
+ * unordered pair. */
 rlc_status rlc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, float wmin,
                         float wmax, uint64_t seed, rlc_csr *out);
 /* Road-like grid: rows x cols 4-neighbour lattice, each lattice edge dropped with probability

@@ -83,8 +83,8 @@ rlc_status build_csr(uint64_t n, uint64_t count, Gen gen, WeightFn wfn, bool uni
   for (uint64_t u = 0; u < n; ++u) { off[u] = m; m += cur[u]; }
   off[n] = m;
   uint32_t *edg = (uint32_t *)std::malloc(std::max<uint64_t>(m, 1) * sizeof(uint32_t));
-  float *wei = (float *)std::malloc(std::max<uint64_t>(m, 1) * sizeof(float));
-  if (!edg || !wei) {
+  float *wei = unit ? nullptr : (float *)std::malloc(std::max<uint64_t>(m, 1) * sizeof(float));
+  if (!edg || (!unit && !wei)) {
     std::free(deg); std::free(cur); std::free(tmp); std::free(off); std::free(edg); std::free(wei);
     return rlc::fail(RLC_ERR_OUT_OF_MEMORY, "generator: output arrays");
   }
@@ -95,7 +95,7 @@ rlc_status build_csr(uint64_t n, uint64_t count, Gen gen, WeightFn wfn, bool uni
     for (uint64_t j = 0; j < d; ++j) {
       uint32_t v = src[j];
       edg[o + j] = v;
-      wei[o + j] = unit ? 1.0f : wfn((uint32_t)u, v);
+      if (!unit) wei[o + j] = wfn((uint32_t)u, v);
     }
   }
   std::free(deg); std::free(cur); std::free(tmp);
@@ -211,13 +211,13 @@ rlc_status rlc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b
   const float wspan = wmax - wmin;
   const uint64_t wseed = mix64(seed ^ 0x5157A1D3ull);
   auto wfn = [=](uint32_t u, uint32_t v) {
+    if (unit) return wmin;
     uint32_t lo = u < v ? u : v, hi = u < v ? v : u;
     double f = double(hash3(wseed, lo, hi) >> 40) * 0x1.0p-24;
     return (float)(double(wmin) + double(wspan) * f);
   };
-  rlc_status st = build_csr(n, count, gen, wfn, unit, out);
-  if (st == RLC_OK && unit && wmin != 1.0f)
-    for (uint64_t i = 0; i < out->m; ++i) out->weights[i] = wmin;
+  const bool ones = unit && wmin == 1.0f;
+  rlc_status st = build_csr(n, count, gen, wfn, ones, out);  // constant weights other than 1 are stored
   return st;
 }
 

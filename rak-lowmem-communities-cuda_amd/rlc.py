@@ -199,33 +199,54 @@ class Graph:
         return np.diff(self.offsets)
 
 
-def _take_csr(c: Csr, unit: bool) -> Graph:
+class _CsrOwner:
+    """Releases a generator CSR when the last numpy view of its buffers is gone."""
+
+    def __init__(self, c: Csr):
+        self.c = c
+
+    def __del__(self):
+        try:
+            lib().rlc_csr_release(C.byref(self.c))
+        except Exception:
+            pass
+
+
+def _view(ptr, ctype, count, dtype, owner):
+    """numpy view of count elements at a C pointer; the view keeps `owner` alive."""
+    if count == 0:
+        return np.zeros(0, dtype)
+    buf = (ctype * count).from_address(C.cast(ptr, C.c_void_p).value)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype)
+
+
+def _take_csr(c: Csr) -> Graph:
+    """numpy views of the generator's buffers (no copy, so the largest graphs are held once)."""
     n, m = int(c.n), int(c.m)
-    off = np.ctypeslib.as_array(c.offsets, shape=(n + 1,)).copy()
-    edg = np.ctypeslib.as_array(c.edges, shape=(m,)).copy() if m else np.zeros(0, np.uint32)
-    wei = None
-    if not unit:
-        wei = np.ctypeslib.as_array(c.weights, shape=(m,)).copy() if m else np.zeros(0, np.float32)
-    lib().rlc_csr_release(C.byref(c))
+    owner = _CsrOwner(c)
+    off = _view(c.offsets, C.c_uint64, n + 1, np.uint64, owner)
+    edg = _view(c.edges, C.c_uint32, m, np.uint32, owner)
+    wei = _view(c.weights, C.c_float, m, np.float32, owner) if c.weights else None
     return Graph(off, edg, wei)
 
 
 def gen_sbm(n: int, blocks: int, degree: float, intra: float, seed: int) -> Graph:
     c = Csr()
     _check(lib().rlc_gen_sbm(n, blocks, degree, intra, seed, C.byref(c)))
-    return _take_csr(c, unit=True)
+    return _take_csr(c)
 
 
 def gen_rmat(scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, wmin=1.0, wmax=1.0, seed=1) -> Graph:
     out = Csr()
     _check(lib().rlc_gen_rmat(scale, edge_factor, a, b, c, wmin, wmax, seed, C.byref(out)))
-    return _take_csr(out, unit=(wmin == wmax == 1.0))
+    return _take_csr(out)
 
 
 def gen_grid(rows: int, cols: int, drop: float, shortcut: float, seed: int) -> Graph:
     c = Csr()
     _check(lib().rlc_gen_grid(rows, cols, drop, shortcut, seed, C.byref(c)))
-    return _take_csr(c, unit=True)
+    return _take_csr(c)
 
 
 class GraphSet:
