@@ -82,7 +82,7 @@ typedef struct rlc_options {
                              chunks, one wavefront per chunk (MG8; 256 lanes per vertex for BM).
                              Non-decreasing; all zero = the library defaults */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
-  int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: blocks of 64
+  int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: windows of 8
                              consecutive ids go round-robin into S segments, run one after the other so that
                              later segments see the labels written by earlier ones); 1 = a single
                              sweep. SYNC: ignored */
@@ -192,8 +192,9 @@ typedef struct rlc_batch_tuning {
                             merge_parts (4, 16 or 64) exactly as in rlc_run */
   int32_t interleave;    /* ASYNC: vertex sets of the 8 sub-sweeps of a 512-vertex row: 0 = 64
                             consecutive vertices, 1 = every 8th vertex, 2 = 8 vertices of each
-                            64-vertex window with a residue that shifts by window [0]. Changes the
-                            asynchronous result, not its speed class; the autotuner keeps it */
+                            64-vertex window with a residue that shifts by window, 3 = every 8th
+                            window of 8 consecutive vertices [3]. Changes the asynchronous result,
+                            not its speed class; the autotuner keeps it */
   int32_t reserved[2];   /* must be zero */
 } rlc_batch_tuning;
 

@@ -52,7 +52,7 @@ rlc_status rlc::check_tuning(const rlc_batch_tuning *t) {
     return fail(RLC_ERR_INVALID_ARGUMENT, "tuning degrees must be >= 0");
   if (t->group_lanes != 4 && t->group_lanes != 16)
     return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.group_lanes must be 4 or 16");
-  if (t->interleave < 0 || t->interleave > 2) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.interleave must be 0, 1 or 2");
+  if (t->interleave < 0 || t->interleave > 3) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.interleave must be 0..3");
   for (int r : t->reserved)
     if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "tuning.reserved must be zero");
   return RLC_OK;
@@ -195,7 +195,7 @@ void rlc_batch_tuning_default(rlc_batch_tuning *t) {
   t->group_degree = 32;
   t->group_lanes = 4;
   t->wave_degree = 256;
-  t->interleave = 0;
+  t->interleave = 3;
 }
 
 uint64_t rlc_batch_footprint(uint64_t n, uint64_t m, int32_t weighted, int32_t sync) {

@@ -48,14 +48,14 @@ __device__ __forceinline__ void block_stats(unsigned long long *out, unsigned lo
 // ---- per-vertex choice -------------------------------------------------------------------------
 // One lane scans vertex u sequentially (CSR order); MG8 candidates are rescanned by gathering
 // the labels again (the labels a vertex reads do not change during a synchronous step).
-template <bool MG, bool UNIT>
+template <bool MG, bool UNIT, bool FIRST_SEEN>
 __device__ __forceinline__ void choose_seq(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t b, uint64_t e,
                                            uint32_t &cb, float &wb) {
   if (MG) {
     MG8 sk;
     sk.clear();
     for_edges<UNIT, 1>(g, lab, u, b, e, [&](uint32_t c, float w) { sk.accumulate(c, w); });
-    if (!certain_argmax(sk.c, sk.w, sk.dsum, e - b, cb, wb)) {
+    if (!certain_argmax<FIRST_SEEN>(sk.c, sk.w, sk.dsum, e - b, cb, wb)) {
       float acc[kSlots];
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
@@ -63,7 +63,7 @@ __device__ __forceinline__ void choose_seq(const DevGraph &g, const uint32_t *la
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) acc[j] = sk.c[j] == c ? acc[j] + w : acc[j];
       });
-      argmax8(sk.c, acc, cb, wb);
+      argmax8<FIRST_SEEN>(sk.c, acc, cb, wb);
     }
   } else {
     uint32_t mc = kEmpty;
@@ -78,7 +78,7 @@ __device__ __forceinline__ void choose_seq(const DevGraph &g, const uint32_t *la
 // are merged by group_merge_tree (part p absorbs p+k), the decrement weights summed by the
 // butterfly, the candidates' exact weights summed by the same butterfly. Every lane of the group
 // returns the same (cb, wb).
-template <bool MG, bool UNIT, int W>
+template <bool MG, bool UNIT, int W, bool FIRST_SEEN>
 __device__ __forceinline__ void choose_group(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t b, uint64_t e,
                                              int gl, uint32_t &cb, float &wb) {
   const int leader = lane_id() & ~(W - 1);
@@ -97,7 +97,7 @@ __device__ __forceinline__ void choose_group(const DevGraph &g, const uint32_t *
     float dtot = sk.dsum;
 #pragma unroll
     for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
-    if (!certain_argmax(cand, cw, dtot, e - b, cb, wb)) {
+    if (!certain_argmax<FIRST_SEEN>(cand, cw, dtot, e - b, cb, wb)) {
       float acc[kSlots];
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
@@ -106,7 +106,7 @@ __device__ __forceinline__ void choose_group(const DevGraph &g, const uint32_t *
         for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == c ? acc[j] + w : acc[j];
       });
       group_sum8<W>(acc);
-      argmax8(cand, acc, cb, wb);
+      argmax8<FIRST_SEEN>(cand, acc, cb, wb);
     }
   } else {
     uint32_t mc = kEmpty;
@@ -170,7 +170,7 @@ __device__ __forceinline__ uint32_t run_groups(const BatchArgs &a, unsigned long
     uint32_t cb = 0;
     if (has) {
       float wb = 0.0f;
-      choose_group<MG, UNIT, W>(g, s.lab, u, vb, ve, gl, cb, wb);
+      choose_group<MG, UNIT, W, !SYNC>(g, s.lab, u, vb, ve, gl, cb, wb);
       moved = wb > 0.0f && cb != vd && !(pickless && cb > vd);
       if (gl == 0) s.aff[u] = 0;
       if (!SYNC && moved) {
@@ -197,6 +197,8 @@ __device__ __forceinline__ uint32_t run_groups(const BatchArgs &a, unsigned long
 //   mixed   (interleave 2)  8 vertices of each of the row's 8 windows, the window's residue
 //                           mod 8 shifted by the window: 512 r + 64 w + 8 k + ((sub + w) & 7),
 //                           lane = 8 w + k
+//   windows of 8 (interleave 3)  the 8-vertex windows j = 8 k + sub: 512 r + 64 k + 8 sub + i,
+//                           lane = 8 k + i
 __device__ __forceinline__ uint64_t row_vertex(uint32_t interleave, uint32_t sweeps, uint64_t r, uint32_t sub,
                                                int lane) {
   if (sweeps == 1) return r * 64u + (uint64_t)lane;
@@ -206,6 +208,7 @@ __device__ __forceinline__ uint64_t row_vertex(uint32_t interleave, uint32_t swe
     const uint32_t w = (uint32_t)lane >> 3, k = (uint32_t)lane & 7u;
     return base + 64u * w + 8u * k + ((sub + w) & 7u);
   }
+  if (interleave == 3) return base + 64u * ((uint32_t)lane >> 3) + 8u * sub + ((uint32_t)lane & 7u);
   return base + 64u * sub + (uint64_t)lane;
 }
 
@@ -245,9 +248,9 @@ __device__ __forceinline__ uint32_t process_item(const BatchArgs &a, const Batch
     bool moved = false;
     uint32_t cb = 0;
     float wb = 0.0f;
-    staged_choose<MG, UNIT, kStageCap>(g, s.lab, L, cls == 0 && deg <= (uint64_t)kStageCap, u, b, e, cb, wb);
+    staged_choose<MG, UNIT, kStageCap, !SYNC>(g, s.lab, L, cls == 0 && deg <= (uint64_t)kStageCap, u, b, e, cb, wb);
     if (cls == 0) {
-      if (deg > (uint64_t)kStageCap) choose_seq<MG, UNIT>(g, s.lab, u, b, e, cb, wb);
+      if (deg > (uint64_t)kStageCap) choose_seq<MG, UNIT, !SYNC>(g, s.lab, u, b, e, cb, wb);
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       s.aff[u] = 0;
       if (!SYNC && moved) {

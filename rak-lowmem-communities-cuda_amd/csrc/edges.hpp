@@ -192,9 +192,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Lanes with `part` set (their vertex u has edges [b, e) with e - b <= CAP) get the sketch's
-// choice (cb, wb); MG = Misra-Gries 8 slots with certified-or-rescanned argmax, else Boyer-Moore.
+// choice (cb, wb); MG = Misra-Gries 8 slots with certified-or-rescanned argmax (ties: see
+// argmax8), else Boyer-Moore.
 // Called by the whole wave (all lanes, converged).
-template <bool MG, bool UNIT, int CAP>
+template <bool MG, bool UNIT, int CAP, bool FIRST_SEEN>
 __device__ __forceinline__ void staged_choose(const DevGraph &g, const uint32_t *lab, StageLds<UNIT, CAP> &L, bool part,
                                               uint32_t u, uint64_t b, uint64_t e, uint32_t &cb, float &wb) {
   const int lane = lane_id();
@@ -260,7 +261,7 @@ __device__ __forceinline__ void staged_choose(const DevGraph &g, const uint32_t 
           const uint32_t c = L.lab[j];
           if (c != kEmpty) sk.accumulate(c, UNIT ? 1.0f : L.w[j]);
         }
-        if (!certain_argmax(sk.c, sk.w, sk.dsum, (uint64_t)(s1 - s0), cb, wb)) {
+        if (!certain_argmax<FIRST_SEEN>(sk.c, sk.w, sk.dsum, (uint64_t)(s1 - s0), cb, wb)) {
           float acc[kSlots];
 #pragma unroll
           for (int q = 0; q < kSlots; ++q) acc[q] = 0.0f;
@@ -272,7 +273,7 @@ __device__ __forceinline__ void staged_choose(const DevGraph &g, const uint32_t 
               for (int q = 0; q < kSlots; ++q) acc[q] = sk.c[q] == c ? acc[q] + w : acc[q];
             }
           }
-          argmax8(sk.c, acc, cb, wb);
+          argmax8<FIRST_SEEN>(sk.c, acc, cb, wb);
         }
       } else {
         uint32_t mc = kEmpty;

@@ -43,7 +43,7 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
   return RLC_OK;
 }
 
-uint64_t segment_of(uint64_t u, uint64_t S) { return S <= 1 ? 0 : (u / 64) % S; }
+uint64_t segment_of(uint64_t u, uint64_t S) { return S <= 1 ? 0 : (u / kSegmentWindow) % S; }
 
 rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
                         int segments, Layout &L) {

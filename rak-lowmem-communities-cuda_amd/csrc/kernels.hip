@@ -68,15 +68,15 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
         auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
         for_edges<UNIT, 1, decltype(acc_fn), RESCAN>(g, s.lab, u, b, e, acc_fn, s.lbuf);
         if (!RESCAN) {
-          argmax8(sk.c, sk.w, cb, wb);
-        } else if (!certain_argmax(sk.c, sk.w, sk.dsum, deg, cb, wb)) {  // rescan
+          argmax8<!SYNC>(sk.c, sk.w, cb, wb);
+        } else if (!certain_argmax<!SYNC>(sk.c, sk.w, sk.dsum, deg, cb, wb)) {  // rescan
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) sk.w[j] = 0.0f;
           for_labels<UNIT, 1>(g, s.lbuf, b, e, [&](uint32_t c, float w) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) sk.w[j] = sk.c[j] == c ? sk.w[j] + w : sk.w[j];
           });
-          argmax8(sk.c, sk.w, cb, wb);
+          argmax8<!SYNC>(sk.c, sk.w, cb, wb);
         }
       } else {
         uint32_t mc = kEmpty;
@@ -224,8 +224,8 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
         }
         float acc[kSlots];
         if (!RESCAN) {
-          argmax8(cand, cw, cb, wb);
-        } else if (certain_argmax(cand, cw, dtot, deg, cb, wb)) {  // the rescan cannot change the choice
+          argmax8<!SYNC>(cand, cw, cb, wb);
+        } else if (certain_argmax<!SYNC>(cand, cw, dtot, deg, cb, wb)) {  // the rescan cannot change the choice
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = cw[j];
         } else {
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = sh_tot[j];
         }
-        argmax8(cand, acc, cb, wb);
+        argmax8<!SYNC>(cand, acc, cb, wb);
         }
       } else {
         uint32_t mc = kEmpty;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_merge(DevGraph g, DevState s, 
       const uint64_t deg = g.off[(uint64_t)u + 1] - g.off[u];
       uint32_t cb;
       float wb;
-      const bool sure = !rescan || certain_argmax(acc.c, acc.w, d, deg, cb, wb);
+      const bool sure = !rescan || certain_argmax<true>(acc.c, acc.w, d, deg, cb, wb);
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) { cp.cand[slot * kSlots + j] = acc.c[j]; cp.cw[slot * kSlots + j] = acc.w[j]; }
       cp.vdec[slot] = sure ? 0u : 1u;  // rescan needed
@@ -461,9 +461,9 @@ __global__ __launch_bounds__(kBlock) void k_chunk_choose(DevGraph g, DevState s,
     uint32_t cb;
     float wb;
     if (cp.vdec[slot]) {
-      argmax8(cand, tot, cb, wb);
+      argmax8<true>(cand, tot, cb, wb);
     } else {  // certified in the merge pass: the first heaviest slot, beyond the others by > D
-      (void)certain_argmax(cand, tot, 0.0f, 0, cb, wb);
+      (void)certain_argmax<true>(cand, tot, 0.0f, 0, cb, wb);
     }
     if (lane == 0) {
       const uint32_t d = s.lab[u];

@@ -80,31 +80,59 @@ __device__ __forceinline__ void max_from(const uint32_t (&c)[kSlots], const floa
   if constexpr (I + 1 < kSlots) max_from<I + 1>(c, w, cb, wb, top);
 }
 
-// First slot of strictly largest weight, from (label 0, weight 0) (inc/rakLowmem.hxx:164-171).
+// The same, over slots I, I-1, .., 0 (the highest slot wins ties).
+template <int I>
+__device__ __forceinline__ void max_down_from(const uint32_t (&c)[kSlots], const float (&w)[kSlots], uint32_t &cb,
+                                              float &wb, int &top) {
+  const bool gt = w[I] > wb;
+  cb = gt ? c[I] : cb;
+  wb = gt ? w[I] : wb;
+  top = gt ? I : top;
+  if constexpr (I > 0) max_down_from<I - 1>(c, w, cb, wb, top);
+}
+
+// Slot of strictly largest weight, from (label 0, weight 0). FIRST_SEEN = false: the first slot
+// wins ties, the reference CPU rule (inc/rakLowmem.hxx:164-171); with its highest-free-slot
+// insertion that is the label seen last. FIRST_SEEN = true: the highest slot wins ties, i.e. the
+// label seen first, as the reference GPU kernels choose (lowest free slot, tree argmax keeping
+// the lower index, inc/sketchCuda.hxx:86-87, :190-201) and as GVE-LPA does (inc/rak.hxx:219-220).
+template <bool FIRST_SEEN = false>
 __device__ __forceinline__ void argmax8(const uint32_t (&c)[kSlots], const float (&w)[kSlots], uint32_t &cb,
                                         float &wb) {
   cb = 0;
   wb = 0.0f;
   int top = 0;
-  max_from<0>(c, w, cb, wb, top);
+  if (FIRST_SEEN) max_down_from<kSlots - 1>(c, w, cb, wb, top);
+  else max_from<0>(c, w, cb, wb, top);
 }
 
 // The reference's choice after a rescan, when it is already determined by the sketch: with no
 // decrement (D == 0) the weights are exact; otherwise the first heaviest slot must beat every
 // other slot by more than D plus a rounding margin (the rescan sums lie in [w, w + D]). Returns
 // false when a rescan is needed. deg bounds the number of float additions behind each weight.
+template <bool FIRST_SEEN = false>
 __device__ __forceinline__ bool certain_argmax(const uint32_t (&c)[kSlots], const float (&w)[kSlots], float D,
                                                uint64_t deg, uint32_t &cb, float &wb) {
   if (D == 0.0f) {
-    argmax8(c, w, cb, wb);
+    argmax8<FIRST_SEEN>(c, w, cb, wb);
     return true;
   }
-  // first heaviest slot (weights are >= 0 and D > 0 here, so some slot is non-zero or the
-  // test below fails) and the largest other weight
-  int top = 0;
-  float wt = w[0];
-  uint32_t ct = c[0];
-  max_from<1>(c, w, ct, wt, top);
+  // heaviest slot (weights are >= 0 and D > 0 here, so some slot is non-zero or the test below
+  // fails) and the largest other weight; a tie with another slot never passes the test
+  int top;
+  float wt;
+  uint32_t ct;
+  if (FIRST_SEEN) {
+    top = kSlots - 1;
+    wt = w[kSlots - 1];
+    ct = c[kSlots - 1];
+    max_down_from<kSlots - 2>(c, w, ct, wt, top);
+  } else {
+    top = 0;
+    wt = w[0];
+    ct = c[0];
+    max_from<1>(c, w, ct, wt, top);
+  }
   float second = 0.0f;
 #pragma unroll
   for (int i = 0; i < kSlots; ++i) second = (i != top && w[i] > second) ? w[i] : second;
