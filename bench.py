@@ -345,7 +345,9 @@ def main():
     opts = rlc.options(mode=rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
                        group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps)
     ctx = rlc.Context(0)
+    t0 = time.time()
     dg = ctx.upload(g)
+    log(f"upload (device layout): {time.time() - t0:.2f}s")
     for _ in range(args.warmup):
         dg.run(opts, labels=False)
     rlc.lib()  # loaded

@@ -14,6 +14,9 @@
  *   - rlc_modularity        <- modularityBy + communities (quality gate of main.cxx)
  *                                                                     inc/properties.hxx:177-192, :352-364;
  *                                                                     main.cxx:44-48, :61, :69
+ *   - rlc_mtx_read,         <- readMtxOmpW + symmetrizeOmpU + the DiGraph's sorted, deduplicated
+ *     rlc_graph_from_edges     adjacency (inc/mtx.hxx:151-187, inc/symmetrize.hxx:43-69,
+ *                              inc/_bitset.hxx:245-262), built on the device
  *   - rlc_gen_*             <- no counterpart (the reference reads MTX files, inc/mtx.hxx);
  *                              seeded synthetic inputs for the benchmark configurations.
  *   - rlc_batch_*           <- no counterpart: many graphs packed in one CSR arena, one launch.
@@ -149,6 +152,33 @@ rlc_status rlc_graph_upload32(rlc_context *ctx, uint64_t n, uint64_t m, const ui
                               const uint32_t *edges, const float *weights, rlc_graph **out);
 rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info);
 void rlc_graph_destroy(rlc_graph *g);
+
+/* ---- graph construction (the reference's ingest: inc/mtx.hxx, inc/symmetrize.hxx) ----------- */
+/* A 0-based edge list: edge i is (src[i], dst[i]) with weight weights[i] (weights NULL: 1). */
+typedef struct rlc_edges {
+  uint64_t n;        /* vertices */
+  uint64_t count;    /* edges */
+  uint32_t *src, *dst;
+  float *weights;    /* NULL when not read */
+  int32_t symmetric; /* the file declared itself symmetric (both directions are already listed) */
+  int32_t reserved;
+} rlc_edges;
+/* Read a Matrix Market coordinate file as the reference does (readMtxOmpW, inc/mtx.hxx:151-187):
+ * vertices 1..max(rows, cols) become 0..n-1; each line gives (u, v) and, for a symmetric file,
+ * (v, u) right after it; with weighted, the third column is the weight and 0 (or none) means 1.
+ * Parsed in parallel on the host. Release with rlc_edges_release. */
+rlc_status rlc_mtx_read(const char *path, int32_t weighted, rlc_edges *out);
+void rlc_edges_release(rlc_edges *e);
+/* Build a graph on the device from a host edge list (ids < n): adjacency sorted ascending, a repeated
+ * edge keeps its last weight (DiGraph insertion, inc/_bitset.hxx:245-262); symmetrize = 1 adds
+ * (v, u) for every (u, v) whose reverse is absent, with (u, v)'s weight (symmetrizeOmpU,
+ * inc/symmetrize.hxx:43-69). Self-loops are kept (the scans skip them). The result is the same
+ * graph rlc_graph_upload makes of that CSR. */
+rlc_status rlc_graph_from_edges(rlc_context *ctx, uint64_t n, uint64_t count, const uint32_t *src,
+                                const uint32_t *dst, const float *weights, int32_t symmetrize, rlc_graph **out);
+/* The graph's CSR in caller order (offsets n+1, edges m, weights m; weights may be NULL; use
+ * rlc_graph_info_get for n and m). */
+rlc_status rlc_graph_csr_get(const rlc_graph *g, uint64_t *offsets, uint32_t *edges, float *weights);
 
 /* ---- label propagation --------------------------------------------------------------------- */
 /* Run LPA on an uploaded graph. init_labels (host, n entries, each != RLC_EMPTY_LABEL) may be

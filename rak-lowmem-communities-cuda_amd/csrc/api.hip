@@ -17,6 +17,7 @@
 
 #include "context.hpp"
 #include "host.hpp"
+#include "ingest.hpp"
 #include "lpa.hpp"
 
 using namespace rlc;
@@ -358,49 +359,84 @@ rlc_status rlc_device_memory(rlc_context *ctx, uint64_t *free_bytes, uint64_t *t
   return RLC_OK;
 }
 
-static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::vector<uint64_t> &&off,
-                                const uint32_t *edges, const float *weights, rlc_graph **out) {
-  uint64_t md = 0;
-  rlc_status st = validate_csr(n, m, off.data(), edges, weights, &md);
-  if (st != RLC_OK) return st;
-  Layout L;
-  if ((st = build_layout(n, m, off.data(), edges, weights, segments_for(n), L)) != RLC_OK) return st;
-  std::vector<uint64_t>().swap(off);
+// A graph object (stream, events, state arrays) around a device CSR: the layout is built on the
+// device and takes the CSR's place. csr is released in every case.
+static rlc_status graph_from_device_csr(rlc_context *ctx, rlc_graph *g, DevCsr &csr, rlc_graph **out) {
+  auto bail = [&](rlc_status s) {
+    csr.release();
+    free_graph(g);
+    return s;
+  };
+  DevLayout L;
+  rlc_status st = device_layout(g->stream, csr, segments_for(csr.n), L);
+  csr.release();
+  if (st != RLC_OK) return bail(st);
+  g->n = L.csr.n;
+  g->m = L.csr.m;
+  g->max_degree = L.max_degree;
+  g->unit = L.csr.wei == nullptr;
+  g->d_off = L.csr.off;
+  g->d_edg = L.csr.edg;
+  g->d_wei = L.csr.wei;
+  g->d_perm = L.perm;
+  g->h_off.swap(L.h_off);
+  g->h_perm.swap(L.h_perm);
+  g->h_seg.swap(L.h_seg);
+  const uint64_t n = g->n;
+  if ((st = dev_alloc(&g->d_lab, n)) != RLC_OK || (st = dev_alloc(&g->d_aff, n)) != RLC_OK ||
+      (st = dev_alloc(&g->d_ctr, 1)) != RLC_OK)
+    return bail(st);
+  *out = g;
+  return RLC_OK;
+}
+
+static rlc_status new_graph(rlc_context *ctx, rlc_graph **out) {
   HIP_TRY(hipSetDevice(ctx->device));
   rlc_graph *g = new rlc_graph();
   g->ctx = ctx;
-  g->n = n;
-  g->m = m;
-  g->max_degree = md;
-  g->unit = weights == nullptr;
-  auto bail = [&](rlc_status s) { free_graph(g); return s; };
+  auto bail = [&](rlc_status s) {
+    free_graph(g);
+    return s;
+  };
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "hipStreamCreate failed"));
   for (auto &e : g->ev)
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
   for (auto &e : g->evm)
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
-  if ((st = dev_alloc(&g->d_off, n + 1)) != RLC_OK) return bail(st);
-  if ((st = dev_alloc(&g->d_edg, m)) != RLC_OK) return bail(st);
-  if (!g->unit && (st = dev_alloc(&g->d_wei, m)) != RLC_OK) return bail(st);
-  if ((st = dev_alloc(&g->d_perm, n)) != RLC_OK) return bail(st);
-  if ((st = dev_alloc(&g->d_lab, n)) != RLC_OK) return bail(st);
-  if ((st = dev_alloc(&g->d_aff, n)) != RLC_OK) return bail(st);
-  if ((st = dev_alloc(&g->d_ctr, 1)) != RLC_OK) return bail(st);
+  *out = g;
+  return RLC_OK;
+}
+
+static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::vector<uint64_t> &&off,
+                                const uint32_t *edges, const float *weights, rlc_graph **out) {
+  uint64_t md = 0;
+  rlc_status st = validate_csr(n, m, off.data(), edges, weights, &md);
+  if (st != RLC_OK) return st;
+  rlc_graph *g = nullptr;
+  if ((st = new_graph(ctx, &g)) != RLC_OK) return st;
+  DevCsr csr;
+  csr.n = n;
+  csr.m = m;
   auto cp = [&](void *dst, const void *src, size_t bytes) {
     return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g->stream) : hipSuccess;
   };
-  if (cp(g->d_off, L.off.data(), (n + 1) * sizeof(uint64_t)) != hipSuccess ||
-      cp(g->d_edg, L.edg.data(), m * sizeof(uint32_t)) != hipSuccess ||
-      (!g->unit && cp(g->d_wei, L.wei.data(), m * sizeof(float)) != hipSuccess) ||
-      cp(g->d_perm, L.perm.data(), n * sizeof(uint32_t)) != hipSuccess ||
-      hipStreamSynchronize(g->stream) != hipSuccess)
-    return bail(fail(RLC_ERR_DEVICE, "graph upload copy failed"));
-  g->h_off.swap(L.off);
-  g->h_perm.swap(L.perm);
-  g->h_seg.swap(L.seg);
-  *out = g;
-  return RLC_OK;
+  if ((st = dev_alloc(&csr.off, n + 1)) != RLC_OK || (st = dev_alloc(&csr.edg, m)) != RLC_OK ||
+      (weights && (st = dev_alloc(&csr.wei, m)) != RLC_OK)) {
+    csr.release();
+    free_graph(g);
+    return st;
+  }
+  if (cp(csr.off, off.data(), (n + 1) * sizeof(uint64_t)) != hipSuccess ||
+      cp(csr.edg, edges, m * sizeof(uint32_t)) != hipSuccess ||
+      (weights && cp(csr.wei, weights, m * sizeof(float)) != hipSuccess) ||
+      hipStreamSynchronize(g->stream) != hipSuccess) {
+    csr.release();
+    free_graph(g);
+    return fail(RLC_ERR_DEVICE, "graph upload copy failed");
+  }
+  std::vector<uint64_t>().swap(off);
+  return graph_from_device_csr(ctx, g, csr, out);
 }
 
 rlc_status rlc_graph_upload(rlc_context *ctx, uint64_t n, uint64_t m, const uint64_t *offsets, const uint32_t *edges,
@@ -439,6 +475,78 @@ rlc_status rlc_graph_info_get(const rlc_graph *g, rlc_graph_info *info) {
 }
 
 void rlc_graph_destroy(rlc_graph *g) { free_graph(g); }
+
+rlc_status rlc_graph_from_edges(rlc_context *ctx, uint64_t n, uint64_t count, const uint32_t *src,
+                                const uint32_t *dst, const float *weights, int32_t symmetrize, rlc_graph **out) {
+  clear_error();
+  if (!ctx || !out || (count && (!src || !dst))) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_from_edges: null argument");
+  *out = nullptr;
+  if (n >= 0xFFFFFFFFull) return fail(RLC_ERR_INVALID_GRAPH, "graph: n must be below 2^32-1");
+  if (symmetrize != 0 && symmetrize != 1) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_from_edges: symmetrize must be 0 or 1");
+  int bad = 0;
+#pragma omp parallel for reduction(| : bad)
+  for (int64_t i = 0; i < (int64_t)count; ++i) {
+    bad |= src[i] >= n || dst[i] >= n;
+    if (weights) bad |= !(weights[i] >= 0.0f) || !(weights[i] < __builtin_huge_valf()) ? 2 : 0;
+  }
+  if (bad & 1) return fail(RLC_ERR_INVALID_GRAPH, "graph: an edge endpoint is >= n");
+  if (bad & 2) return fail(RLC_ERR_INVALID_GRAPH, "graph: weights must be finite and non-negative");
+  rlc_graph *g = nullptr;
+  rlc_status st = new_graph(ctx, &g);
+  if (st != RLC_OK) return st;
+  uint32_t *ds = nullptr, *dd = nullptr;
+  float *dw = nullptr;
+  auto done = [&](rlc_status s) {
+    if (ds) (void)hipFree(ds);
+    if (dd) (void)hipFree(dd);
+    if (dw) (void)hipFree(dw);
+    return s;
+  };
+  if ((st = dev_alloc(&ds, count)) != RLC_OK || (st = dev_alloc(&dd, count)) != RLC_OK ||
+      (weights && (st = dev_alloc(&dw, count)) != RLC_OK)) {
+    free_graph(g);
+    return done(st);
+  }
+  if ((count && (hipMemcpyAsync(ds, src, count * 4, hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+                 hipMemcpyAsync(dd, dst, count * 4, hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+                 (weights && hipMemcpyAsync(dw, weights, count * 4, hipMemcpyHostToDevice, g->stream) != hipSuccess))) ||
+      hipStreamSynchronize(g->stream) != hipSuccess) {
+    free_graph(g);
+    return done(fail(RLC_ERR_DEVICE, "rlc_graph_from_edges: copy failed"));
+  }
+  DevCsr csr;
+  if ((st = device_csr_from_edges(g->stream, n, count, ds, dd, dw, symmetrize != 0, csr)) != RLC_OK) {
+    free_graph(g);
+    return done(st);
+  }
+  done(RLC_OK);
+  return graph_from_device_csr(ctx, g, csr, out);
+}
+
+rlc_status rlc_graph_csr_get(const rlc_graph *g, uint64_t *offsets, uint32_t *edges, float *weights) {
+  clear_error();
+  if (!g || !offsets || (g->m && !edges)) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_graph_csr_get: null argument");
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  const uint64_t n = g->n, m = g->m;
+  std::vector<uint32_t> e(m);
+  std::vector<float> w(weights && !g->unit ? m : 0);
+  if (m) HIP_TRY(hipMemcpy(e.data(), g->d_edg, m * 4, hipMemcpyDeviceToHost));
+  if (!w.empty()) HIP_TRY(hipMemcpy(w.data(), g->d_wei, m * 4, hipMemcpyDeviceToHost));
+  // rows back in caller order, targets back to caller ids (row order inside each row is kept)
+  std::vector<uint64_t> pos(n);
+  for (uint64_t p = 0; p < n; ++p) pos[g->h_perm[p]] = p;
+  offsets[0] = 0;
+  for (uint64_t u = 0; u < n; ++u) offsets[u + 1] = offsets[u] + (g->h_off[pos[u] + 1] - g->h_off[pos[u]]);
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t u = 0; u < (int64_t)n; ++u) {
+    const uint64_t p = pos[u], b = g->h_off[p], d = g->h_off[p + 1] - b, o = offsets[u];
+    for (uint64_t j = 0; j < d; ++j) {
+      edges[o + j] = g->h_perm[e[b + j]];
+      if (weights) weights[o + j] = w.empty() ? 1.0f : w[b + j];
+    }
+  }
+  return RLC_OK;
+}
 
 const uint32_t *rlc_graph_labels_device(const rlc_graph *g) { return g ? g->d_lab : nullptr; }
 

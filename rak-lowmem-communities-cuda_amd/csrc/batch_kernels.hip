@@ -212,14 +212,16 @@ __device__ __forceinline__ uint64_t row_vertex(uint32_t interleave, uint32_t swe
   return base + 64u * sub + (uint64_t)lane;
 }
 
-// Staged edges per wavefront (LDS): 4 KB of labels (and 4 KB of weights when weighted).
-constexpr int kStageCap = 1024;
+// Staged edges per wavefront (LDS): 1024 labels (4 KB), or 512 labels and weights (4 KB) when
+// weighted, so that five workgroups fit a compute unit either way.
 template <bool UNIT>
-using BatchStage = StageLds<UNIT, kStageCap>;
+constexpr int stage_cap() { return UNIT ? 1024 : 512; }
+template <bool UNIT>
+using BatchStage = StageLds<UNIT, stage_cap<UNIT>()>;
 
 // One work item: row r of graph `id` (vb: its first vertex, n: its size), one vertex per lane.
 // GW = 0: every vertex is scanned by one lane (no group or wave class: fewer registers). Lane
-// vertices of degree <= kStageCap are scanned from the wave's LDS stage.
+// vertices of degree <= stage_cap are scanned from the wave's LDS stage.
 // Returns the item's number of moves (wave-uniform).
 template <bool SYNC, bool MG, bool UNIT, int GW>
 __device__ __forceinline__ uint32_t process_item(const BatchArgs &a, const BatchTune &t, uint32_t vb, uint32_t n,
@@ -248,9 +250,10 @@ __device__ __forceinline__ uint32_t process_item(const BatchArgs &a, const Batch
     bool moved = false;
     uint32_t cb = 0;
     float wb = 0.0f;
-    staged_choose<MG, UNIT, kStageCap, !SYNC>(g, s.lab, L, cls == 0 && deg <= (uint64_t)kStageCap, u, b, e, cb, wb);
+    constexpr uint64_t cap = (uint64_t)stage_cap<UNIT>();
+    staged_choose<MG, UNIT, stage_cap<UNIT>(), !SYNC>(g, s.lab, L, cls == 0 && deg <= cap, u, b, e, cb, wb);
     if (cls == 0) {
-      if (deg > (uint64_t)kStageCap) choose_seq<MG, UNIT, !SYNC>(g, s.lab, u, b, e, cb, wb);
+      if (deg > cap) choose_seq<MG, UNIT, !SYNC>(g, s.lab, u, b, e, cb, wb);
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       s.aff[u] = 0;
       if (!SYNC && moved) {

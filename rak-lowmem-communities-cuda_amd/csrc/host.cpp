@@ -45,44 +45,4 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
 
 uint64_t segment_of(uint64_t u, uint64_t S) { return S <= 1 ? 0 : (u / kSegmentWindow) % S; }
 
-rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
-                        int segments, Layout &L) {
-  const uint64_t S = (uint64_t)(segments < 1 ? 1 : segments);
-  try {
-    L.perm.resize(n);
-    L.inv.resize(n);
-    L.off.resize(n + 1);
-    L.edg.resize(m);
-    if (wei) L.wei.resize(m);
-  } catch (...) {
-    return fail(RLC_ERR_OUT_OF_MEMORY, "graph layout: host allocation failed");
-  }
-  uint64_t maxd = 0;
-  for (uint64_t u = 0; u < n; ++u) maxd = std::max<uint64_t>(maxd, off[u + 1] - off[u]);
-  // counting sort by (segment, degree descending), stable in id
-  const uint64_t D = maxd + 1;
-  auto key = [&](uint64_t u) { return segment_of(u, S) * D + (maxd - (off[u + 1] - off[u])); };
-  std::vector<uint64_t> cnt(S * D + 1, 0);
-  for (uint64_t u = 0; u < n; ++u) ++cnt[key(u) + 1];
-  for (uint64_t k = 0; k < S * D; ++k) cnt[k + 1] += cnt[k];
-  L.seg.resize(S + 1);
-  for (uint64_t s = 0; s <= S; ++s) L.seg[s] = cnt[s * D];
-  for (uint64_t u = 0; u < n; ++u) L.perm[cnt[key(u)]++] = (uint32_t)u;
-#pragma omp parallel for schedule(static)
-  for (int64_t p = 0; p < (int64_t)n; ++p) L.inv[L.perm[p]] = (uint32_t)p;
-  L.off[0] = 0;
-  for (uint64_t p = 0; p < n; ++p) {
-    const uint64_t u = L.perm[p];
-    L.off[p + 1] = L.off[p] + (off[u + 1] - off[u]);
-  }
-#pragma omp parallel for schedule(dynamic, 256)
-  for (int64_t p = 0; p < (int64_t)n; ++p) {
-    const uint64_t u = L.perm[p], b = off[u], d = off[u + 1] - b, o = L.off[p];
-    for (uint64_t j = 0; j < d; ++j) L.edg[o + j] = L.inv[edg[b + j]];
-    if (wei)
-      for (uint64_t j = 0; j < d; ++j) L.wei[o + j] = wei[b + j];
-  }
-  return RLC_OK;
-}
-
 }  // namespace rlc

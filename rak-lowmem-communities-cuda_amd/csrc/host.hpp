@@ -20,20 +20,4 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
 constexpr uint64_t kSegmentWindow = 8;
 uint64_t segment_of(uint64_t u, uint64_t S);
 
-// Device layout of a graph: vertices split into `segments` sub-sweep segments by
-// segment_of(id), stored segment after segment, by degree descending inside a segment
-// (ties: ascending id), so that the most referenced labels share cache lines and every degree
-// class is a contiguous range of each segment. perm[p] = original id of the vertex at position
-// p, inv[u] = position of original vertex u, seg[s] = first position of segment s (seg[S] = n).
-// Each row keeps its original edge order; targets are renumbered to positions.
-struct Layout {
-  std::vector<uint32_t> perm, inv;
-  std::vector<uint64_t> seg;
-  std::vector<uint64_t> off;
-  std::vector<uint32_t> edg;
-  std::vector<float> wei;  // empty for unit weights
-};
-rlc_status build_layout(uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg, const float *wei,
-                        int segments, Layout &out);
-
 }  // namespace rlc

@@ -1,0 +1,284 @@
+// Device-side graph construction (gfx950): the CSR from an edge list, and the library's
+// degree-descending segmented layout of a CSR, so that neither step runs on the host.
+//
+// The reference builds its graph on the host (DiGraph: readMtxOmpW, inc/mtx.hxx:151-187;
+// symmetrizeOmpU, inc/symmetrize.hxx:43-69; sorted adjacency with duplicates resolved to the last
+// insertion, inc/_bitset.hxx:245-262) and the CSR in partition order on the host
+// (inc/rakLowmemCuda.hxx:552-569, inc/csr.hxx:126-288). Here both are radix sorts and scans on the
+// device (hipCUB over rocPRIM), plus gathers.
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "host.hpp"
+#include "ingest.hpp"
+
+namespace rlc {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint64_t kBigRow = 1024;  // rows longer than this are copied by a whole wavefront
+
+inline unsigned grid_of(uint64_t items) {
+  const uint64_t b = (items + kBlock - 1) / kBlock;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(b, 65535));
+}
+
+// Device buffer freed on scope exit unless taken.
+template <class T>
+struct Buf {
+  T *p = nullptr;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  rlc_status alloc(uint64_t count) { return dev_alloc(&p, count); }
+  T *take() {
+    T *r = p;
+    p = nullptr;
+    return r;
+  }
+};
+
+int bits_for(uint64_t x) {  // bits needed to represent values <= x
+  int b = 0;
+  while (b < 64 && (x >> b) != 0) ++b;
+  return std::max(b, 1);
+}
+
+__global__ void k_degree_keys(const uint64_t *off, uint64_t n, uint64_t *deg) {
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (uint64_t)gridDim.x * kBlock)
+    deg[u] = off[u + 1] - off[u];
+}
+
+// key(u) = segment(u) * (maxd + 1) + (maxd - deg(u)); sorted stably with u: the order of build_layout
+__global__ void k_layout_keys(const uint64_t *deg, uint64_t n, uint64_t maxd, uint64_t S, uint64_t *key,
+                              uint32_t *ids) {
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u < n; u += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t seg = S <= 1 ? 0 : (u / kSegmentWindow) % S;
+    key[u] = seg * (maxd + 1) + (maxd - deg[u]);
+    ids[u] = (uint32_t)u;
+  }
+}
+
+__global__ void k_inverse_and_degrees(const uint32_t *perm, const uint64_t *deg, uint64_t n, uint32_t *inv,
+                                      uint64_t *deg2) {
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p <= n; p += (uint64_t)gridDim.x * kBlock) {
+    if (p == n) {
+      deg2[n] = 0;
+      continue;
+    }
+    const uint32_t u = perm[p];
+    inv[u] = (uint32_t)p;
+    deg2[p] = deg[u];
+  }
+}
+
+// Rows of at most kBigRow edges: one lane per row (targets renumbered to positions).
+__global__ void k_copy_rows(const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *perm,
+                            const uint32_t *inv, const uint64_t *off2, uint64_t n, uint32_t *edg2, float *wei2) {
+  for (uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < n; p += (uint64_t)gridDim.x * kBlock) {
+    const uint32_t u = perm[p];
+    const uint64_t b = off[u], d = off[u + 1] - b, o = off2[p];
+    if (d > kBigRow) continue;
+    for (uint64_t j = 0; j < d; ++j) edg2[o + j] = inv[edg[b + j]];
+    if (wei)
+      for (uint64_t j = 0; j < d; ++j) wei2[o + j] = wei[b + j];
+  }
+}
+
+// Rows longer than kBigRow, positions [first, last): one wavefront per row.
+__global__ void k_copy_big_rows(const uint64_t *off, const uint32_t *edg, const float *wei, const uint32_t *perm,
+                                const uint32_t *inv, const uint64_t *off2, uint64_t first, uint64_t last,
+                                uint32_t *edg2, float *wei2) {
+  const uint64_t wave = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nw = ((uint64_t)gridDim.x * kBlock) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint64_t p = first + wave; p < last; p += nw) {
+    const uint32_t u = perm[p];
+    const uint64_t b = off[u], d = off[u + 1] - b, o = off2[p];
+    for (uint64_t j = lane; j < d; j += 64) {
+      edg2[o + j] = inv[edg[b + j]];
+      if (wei) wei2[o + j] = wei[b + j];
+    }
+  }
+}
+
+// Edge list -> sort keys: with symmetrize, entries [0, count) are the reverse copies (v, u) and
+// [count, 2 count) the edges themselves, so that after a stable sort the last entry of every
+// (u, v) run is the last given edge (u, v) if there is one, else the last reverse copy.
+__global__ void k_edge_keys(const uint32_t *src, const uint32_t *dst, const float *w, uint64_t count, int sym,
+                            uint64_t *key, float *kw) {
+  const uint64_t base = sym ? count : 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < count; i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t u = src[i], v = dst[i];
+    const float x = w ? w[i] : 1.0f;
+    key[base + i] = (u << 32) | v;
+    if (kw) kw[base + i] = x;
+    if (sym) {
+      key[i] = (v << 32) | u;
+      if (kw) kw[i] = x;
+    }
+  }
+}
+
+__global__ void k_last_of_run(const uint64_t *key, uint64_t E, uint8_t *flag) {
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < E; j += (uint64_t)gridDim.x * kBlock)
+    flag[j] = (j + 1 == E || key[j] != key[j + 1]) ? 1 : 0;
+}
+
+// off[u] = first entry with source >= u (binary search over the sorted keys), edg = low words.
+__global__ void k_offsets(const uint64_t *key, uint64_t m, uint64_t n, uint64_t *off) {
+  for (uint64_t u = (uint64_t)blockIdx.x * kBlock + threadIdx.x; u <= n; u += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t x = u << 32;
+    uint64_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (key[mid] < x) lo = mid + 1;
+      else hi = mid;
+    }
+    off[u] = lo;
+  }
+}
+
+__global__ void k_low_words(const uint64_t *key, uint64_t m, uint32_t *edg) {
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < m; j += (uint64_t)gridDim.x * kBlock)
+    edg[j] = (uint32_t)key[j];
+}
+
+}  // namespace
+
+void DevCsr::release() {
+  if (off) (void)hipFree(off);
+  if (edg) (void)hipFree(edg);
+  if (wei) (void)hipFree(wei);
+  off = nullptr;
+  edg = nullptr;
+  wei = nullptr;
+}
+
+rlc_status device_layout(hipStream_t st, const DevCsr &in, int segments, DevLayout &out) {
+  const uint64_t n = in.n, m = in.m;
+  const uint64_t S = (uint64_t)std::max(segments, 1);
+  rlc_status s;
+  Buf<uint64_t> deg, key, key2, deg2, off2, maxd_d;
+  Buf<uint32_t> ids, perm, inv, edg2;
+  Buf<float> wei2;
+  Buf<uint8_t> tmp;
+  if ((s = deg.alloc(n)) || (s = key.alloc(n)) || (s = key2.alloc(n)) || (s = deg2.alloc(n + 1)) ||
+      (s = off2.alloc(n + 1)) || (s = maxd_d.alloc(1)) || (s = ids.alloc(n)) || (s = perm.alloc(n)) ||
+      (s = inv.alloc(n)) || (s = edg2.alloc(m)) || (in.wei && (s = wei2.alloc(m))))
+    return s;
+  uint64_t maxd = 0;
+  if (n) {
+    k_degree_keys<<<grid_of(n), kBlock, 0, st>>>(in.off, n, deg.p);
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceReduce::Max(nullptr, tb, deg.p, maxd_d.p, n, st));
+    Buf<uint8_t> t1;
+    if ((s = t1.alloc(tb))) return s;
+    HIP_TRY(hipcub::DeviceReduce::Max(t1.p, tb, deg.p, maxd_d.p, n, st));
+    HIP_TRY(hipMemcpyAsync(&maxd, maxd_d.p, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    k_layout_keys<<<grid_of(n), kBlock, 0, st>>>(deg.p, n, maxd, S, key.p, ids.p);
+    const int end_bit = bits_for(S * (maxd + 1) - 1);
+    tb = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, key2.p, ids.p, perm.p, n, 0, end_bit, st));
+    if ((s = tmp.alloc(tb))) return s;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, key2.p, ids.p, perm.p, n, 0, end_bit, st));
+  }
+  k_inverse_and_degrees<<<grid_of(n + 1), kBlock, 0, st>>>(perm.p, deg.p, n, inv.p, deg2.p);
+  {
+    size_t tb = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, deg2.p, off2.p, n + 1, st));
+    Buf<uint8_t> t2;
+    if ((s = t2.alloc(tb))) return s;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(t2.p, tb, deg2.p, off2.p, n + 1, st));
+    out.h_off.resize(n + 1);
+    out.h_perm.resize(n);
+    HIP_TRY(hipMemcpyAsync(out.h_off.data(), off2.p, (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    if (n) HIP_TRY(hipMemcpyAsync(out.h_perm.data(), perm.p, n * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));  // the scan's temporary storage is freed at scope exit
+  }
+  // segment starts: segment s holds the ids u with (u / window) % S == s
+  out.h_seg.assign(S + 1, 0);
+  for (uint64_t w0 = 0; w0 < n; w0 += kSegmentWindow) {
+    const uint64_t seg = S <= 1 ? 0 : (w0 / kSegmentWindow) % S;
+    out.h_seg[seg + 1] += std::min<uint64_t>(kSegmentWindow, n - w0);
+  }
+  for (uint64_t q = 0; q < S; ++q) out.h_seg[q + 1] += out.h_seg[q];
+  // rows: short ones one lane each, then the long ones (each segment starts with them) a wave each
+  if (n) k_copy_rows<<<grid_of(n), kBlock, 0, st>>>(in.off, in.edg, in.wei, perm.p, inv.p, off2.p, n, edg2.p, wei2.p);
+  for (uint64_t q = 0; q < S; ++q) {
+    uint64_t a = out.h_seg[q], b = out.h_seg[q + 1], lo = a, hi = b;
+    while (lo < hi) {  // first position of the segment with degree <= kBigRow
+      const uint64_t mid = lo + (hi - lo) / 2;
+      if (out.h_off[mid + 1] - out.h_off[mid] > kBigRow) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo > a)
+      k_copy_big_rows<<<grid_of((lo - a) * 64), kBlock, 0, st>>>(in.off, in.edg, in.wei, perm.p, inv.p, off2.p, a, lo,
+                                                                 edg2.p, wei2.p);
+    (void)b;
+  }
+  if ((s = check_launch()) != RLC_OK) return s;
+  HIP_TRY(hipStreamSynchronize(st));
+  out.max_degree = maxd;
+  out.csr.n = n;
+  out.csr.m = m;
+  out.csr.off = off2.take();
+  out.csr.edg = edg2.take();
+  out.csr.wei = in.wei ? wei2.take() : nullptr;
+  out.perm = perm.take();
+  return RLC_OK;
+}
+
+rlc_status device_csr_from_edges(hipStream_t st, uint64_t n, uint64_t count, const uint32_t *src, const uint32_t *dst,
+                                 const float *w, bool symmetrize, DevCsr &out) {
+  const uint64_t E = symmetrize ? 2 * count : count;
+  rlc_status s;
+  Buf<uint64_t> key, key2, ukey, off, nsel;
+  Buf<float> kw, kw2, uw;
+  Buf<uint8_t> flag, tmp;
+  Buf<uint32_t> edg;
+  if ((s = key.alloc(E)) || (s = key2.alloc(E)) || (s = ukey.alloc(E)) || (s = off.alloc(n + 1)) ||
+      (s = nsel.alloc(1)) || (s = flag.alloc(E)) || (w && ((s = kw.alloc(E)) || (s = kw2.alloc(E)) || (s = uw.alloc(E)))))
+    return s;
+  uint64_t m = 0;
+  if (E) {
+    k_edge_keys<<<grid_of(count), kBlock, 0, st>>>(src, dst, w, count, symmetrize ? 1 : 0, key.p, kw.p);
+    const int end_bit = 32 + bits_for(n ? n - 1 : 0);
+    size_t tb = 0;
+    if (w) HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key.p, key2.p, kw.p, kw2.p, E, 0, end_bit, st));
+    else HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tb, key.p, key2.p, E, 0, end_bit, st));
+    if ((s = tmp.alloc(tb))) return s;
+    if (w) HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, key.p, key2.p, kw.p, kw2.p, E, 0, end_bit, st));
+    else HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tb, key.p, key2.p, E, 0, end_bit, st));
+    k_last_of_run<<<grid_of(E), kBlock, 0, st>>>(key2.p, E, flag.p);
+    size_t tb2 = 0;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, tb2, key2.p, flag.p, ukey.p, nsel.p, (int64_t)E, st));
+    Buf<uint8_t> t2;
+    if ((s = t2.alloc(tb2))) return s;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(t2.p, tb2, key2.p, flag.p, ukey.p, nsel.p, (int64_t)E, st));
+    if (w) HIP_TRY(hipcub::DeviceSelect::Flagged(t2.p, tb2, kw2.p, flag.p, uw.p, nsel.p, (int64_t)E, st));
+    HIP_TRY(hipMemcpyAsync(&m, nsel.p, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  if ((s = edg.alloc(m))) return s;
+  k_offsets<<<grid_of(n + 1), kBlock, 0, st>>>(ukey.p, m, n, off.p);
+  if (m) k_low_words<<<grid_of(m), kBlock, 0, st>>>(ukey.p, m, edg.p);
+  if ((s = check_launch()) != RLC_OK) return s;
+  HIP_TRY(hipStreamSynchronize(st));
+  out.n = n;
+  out.m = m;
+  out.off = off.take();
+  out.edg = edg.take();
+  out.wei = w ? uw.take() : nullptr;
+  return RLC_OK;
+}
+
+}  // namespace rlc

@@ -63,6 +63,12 @@ class Csr(C.Structure):
                 ("edges", C.POINTER(C.c_uint32)), ("weights", C.POINTER(C.c_float))]
 
 
+class Edges(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("count", C.c_uint64), ("src", C.POINTER(C.c_uint32)),
+                ("dst", C.POINTER(C.c_uint32)), ("weights", C.POINTER(C.c_float)), ("symmetric", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class BatchTuning(C.Structure):
     _fields_ = [("blocks_per_cu", C.c_int32), ("grab", C.c_int32), ("group_degree", C.c_int32),
                 ("group_lanes", C.c_int32), ("wave_degree", C.c_int32), ("interleave", C.c_int32),
@@ -114,6 +120,10 @@ SIGNATURES = [
     ("rlc_graph_upload32", C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, C.POINTER(_P)]),
     ("rlc_graph_info_get", C.c_int, [_P, C.POINTER(GraphInfo)]),
     ("rlc_graph_destroy", None, [_P]),
+    ("rlc_mtx_read", C.c_int, [C.c_char_p, C.c_int32, C.POINTER(Edges)]),
+    ("rlc_edges_release", None, [C.POINTER(Edges)]),
+    ("rlc_graph_from_edges", C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int32, C.POINTER(_P)]),
+    ("rlc_graph_csr_get", C.c_int, [_P, _P, _P, _P]),
     ("rlc_run", C.c_int, [_P, _P, _P, C.POINTER(Options), _P, C.POINTER(RunStats)]),
     ("rlc_graph_labels_device", _P, [_P]),
     ("rlc_graph_vertex_order", C.c_int, [_P, _P]),
@@ -283,6 +293,39 @@ def gen_sbm_set(n, blocks, degree, intra, seeds) -> GraphSet:
     return GraphSet(vs, es, off, edg)
 
 
+class _EdgesOwner:
+    def __init__(self, e: Edges):
+        self.e = e
+
+    def __del__(self):
+        try:
+            lib().rlc_edges_release(C.byref(self.e))
+        except Exception:
+            pass
+
+
+@dataclass
+class EdgeList:
+    """0-based edge list (src[i], dst[i], weights[i] or None) of n vertices."""
+    n: int
+    src: np.ndarray
+    dst: np.ndarray
+    weights: np.ndarray | None
+    symmetric: bool = False
+
+
+def read_mtx(path: str, weighted: bool = False) -> EdgeList:
+    """Matrix Market file -> 0-based edge list, with the reference reader's rules (rlc_mtx_read)."""
+    e = Edges()
+    _check(lib().rlc_mtx_read(os.fsencode(path), 1 if weighted else 0, C.byref(e)))
+    owner = _EdgesOwner(e)
+    k = int(e.count)
+    src = _view(e.src, C.c_uint32, k, np.uint32, owner)
+    dst = _view(e.dst, C.c_uint32, k, np.uint32, owner)
+    w = _view(e.weights, C.c_float, k, np.float32, owner) if e.weights else None
+    return EdgeList(int(e.n), src, dst, w, bool(e.symmetric))
+
+
 def sbm_mix_params(count: int, seed: int, n_min: int = 1000, n_max: int = 1_000_000):
     """Parameters of the batched workload's graph mix: vertex counts log-uniform in
     [n_min, n_max], 4..1024 blocks (at most n/2), expected degree uniform in [4, 32], 70-90% of
@@ -354,6 +397,16 @@ class Context:
             _check(lib().rlc_graph_upload(self.h, g.n, g.m, _ptr(off), _ptr(edg), _ptr(wei), C.byref(h)))
         return DeviceGraph(self, h, g.n)
 
+
+    def upload_edges(self, el: EdgeList, symmetrize: bool = False) -> "DeviceGraph":
+        """Graph built on the device from an edge list (rlc_graph_from_edges)."""
+        h = C.c_void_p()
+        src = np.ascontiguousarray(el.src, dtype=np.uint32)
+        dst = np.ascontiguousarray(el.dst, dtype=np.uint32)
+        w = None if el.weights is None else np.ascontiguousarray(el.weights, dtype=np.float32)
+        _check(lib().rlc_graph_from_edges(self.h, el.n, len(src), _ptr(src), _ptr(dst), _ptr(w), 1 if symmetrize else 0,
+                                          C.byref(h)))
+        return DeviceGraph(self, h, el.n)
 
     def batch(self, vertex_capacity: int, edge_capacity: int, max_graphs: int, weighted: bool = False) -> "Batch":
         h = C.c_void_p()
@@ -536,6 +589,15 @@ class DeviceGraph:
         ini = None if init is None else np.ascontiguousarray(init, dtype=np.uint32)
         _check(lib().rlc_run(self.ctx.h, self.h, _ptr(ini), C.byref(o), _ptr(out), C.byref(st)))
         return RakResult(out, st.iterations, st.time_ms, st.marking_ms, st.init_ms, st.memory_gb, st)
+
+    def csr(self) -> Graph:
+        """The graph's CSR in caller order (rlc_graph_csr_get)."""
+        i = self.info()
+        off = np.empty(int(i.n) + 1, np.uint64)
+        edg = np.empty(int(i.m), np.uint32)
+        wei = None if i.unit_weights else np.empty(int(i.m), np.float32)
+        _check(lib().rlc_graph_csr_get(self.h, _ptr(off), _ptr(edg), _ptr(wei)))
+        return Graph(off, edg, wei)
 
     def vertex_order(self) -> np.ndarray:
         o = np.empty(self.n, np.uint32)

@@ -1,0 +1,129 @@
+"""Graph construction on the device: edge list -> CSR (sorted, deduplicated, symmetrised) and the
+library's degree-descending segmented layout, against Python restatements of the reference's
+rules (DiGraph insertion, inc/_bitset.hxx:245-262; symmetrizeOmpU, inc/symmetrize.hxx:43-69).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rlc
+from graphs import random_weighted
+
+pytestmark = pytest.mark.gpu
+
+
+def expected_csr(n, src, dst, w, symmetrize):
+    """Insert edges in order (a repeated edge keeps its last weight); with symmetrize, add (v, u)
+    with (u, v)'s weight where (v, u) is absent; adjacency ascending."""
+    d = {}
+    for i in range(len(src)):
+        d[(int(src[i]), int(dst[i]))] = 1.0 if w is None else float(w[i])
+    if symmetrize:
+        for (u, v), x in list(d.items()):
+            if (v, u) not in d:
+                d[(v, u)] = x
+    rows = [[] for _ in range(n)]
+    for (u, v), x in d.items():
+        rows[u].append((v, x))
+    off = np.zeros(n + 1, np.uint64)
+    edg, wei = [], []
+    for u in range(n):
+        rows[u].sort()
+        off[u + 1] = off[u] + len(rows[u])
+        edg += [v for v, _ in rows[u]]
+        wei += [x for _, x in rows[u]]
+    return rlc.Graph(off, np.array(edg, np.uint32), None if w is None else np.array(wei, np.float32))
+
+
+def edge_list(n, k, seed, weighted=True, dup=0.2, loops=0.02):
+    r = np.random.default_rng(seed)
+    src = r.integers(0, n, k).astype(np.uint32)
+    dst = r.integers(0, n, k).astype(np.uint32)
+    nd = int(k * dup)  # repeated edges with other weights, and some reversed pairs
+    idx = r.integers(0, k, nd)
+    src = np.concatenate([src, src[idx[: nd // 2]], dst[idx[nd // 2:]]])
+    dst = np.concatenate([dst, dst[idx[: nd // 2]], src[idx[nd // 2:]]])
+    nl = int(k * loops)
+    src[:nl] = dst[:nl]
+    w = r.uniform(0.5, 2.0, len(src)).astype(np.float32) if weighted else None
+    return rlc.EdgeList(n, src, dst, w)
+
+
+def same_graph(a, b):
+    assert np.array_equal(a.offsets, b.offsets)
+    assert np.array_equal(a.edges, b.edges)
+    if b.weights is None:
+        assert a.weights is None
+    else:
+        assert np.array_equal(a.weights, b.weights)
+
+
+@pytest.mark.parametrize("symmetrize", [False, True])
+@pytest.mark.parametrize("weighted", [True, False])
+def test_from_edges_csr(ctx, symmetrize, weighted):
+    el = edge_list(700, 5000, seed=3 + symmetrize, weighted=weighted)
+    with ctx.upload_edges(el, symmetrize) as dg:
+        got = dg.csr()
+    same_graph(got, expected_csr(el.n, el.src, el.dst, el.weights, symmetrize))
+
+
+def test_from_edges_edge_cases(ctx):
+    # isolated vertices at both ends, no edges at all, one vertex
+    for el in [rlc.EdgeList(9, np.array([3, 4], np.uint32), np.array([4, 5], np.uint32), None),
+               rlc.EdgeList(5, np.zeros(0, np.uint32), np.zeros(0, np.uint32), None),
+               rlc.EdgeList(1, np.array([0], np.uint32), np.array([0], np.uint32), None)]:
+        for sym in (False, True):
+            with ctx.upload_edges(el, sym) as dg:
+                same_graph(dg.csr(), expected_csr(el.n, el.src, el.dst, None, sym))
+    with pytest.raises(rlc.RlcError):
+        ctx.upload_edges(rlc.EdgeList(3, np.array([0], np.uint32), np.array([3], np.uint32), None))
+
+
+def test_from_edges_runs_like_upload(ctx):
+    # the same CSR through both paths: identical layout, identical synchronous labels (= oracle)
+    el = edge_list(5000, 40000, seed=9)
+    g = expected_csr(el.n, el.src, el.dst, el.weights, True)
+    opts = rlc.options(mode=rlc.MODE_SYNC)
+    with ctx.upload_edges(el, True) as a, ctx.upload(g) as b:
+        assert np.array_equal(a.vertex_order(), b.vertex_order())
+        ra, rb = a.run(opts), b.run(opts)
+    lab, it, _ = O.lpa_sync(g)
+    assert ra.iterations == rb.iterations == it
+    assert np.array_equal(ra.membership, lab) and np.array_equal(rb.membership, lab)
+
+
+@pytest.mark.parametrize("n", [1000, 70000])
+def test_layout_order(ctx, n):
+    # positions: segment (id / 8) % S, then degree descending, then id (S = min(8, n / 4096))
+    g = random_weighted(n, 6 * n, seed=n)
+    with ctx.upload(g) as dg:
+        order = dg.vertex_order()
+        info = dg.info()
+        same_graph(dg.csr(), g)  # the CSR comes back in caller order
+    S = max(1, min(8, n // 4096))
+    ids = np.arange(n)
+    deg = np.diff(g.offsets).astype(np.int64)
+    exp = np.lexsort((ids, -deg, (ids // 8) % S))
+    assert info.segments == S
+    assert np.array_equal(order, exp.astype(np.uint32))
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_mtx_to_device(ctx, tmp_path, symmetric):
+    r = np.random.default_rng(5)
+    n, k = 3000, 12000
+    u, v = r.integers(1, n + 1, k), r.integers(1, n + 1, k)
+    w = np.round(r.uniform(0.1, 3, k), 3)
+    kind = "symmetric" if symmetric else "general"
+    body = "\n".join(f"{a} {b} {c}" for a, b, c in zip(u, v, w))
+    path = tmp_path / "g.mtx"
+    path.write_text(f"%%MatrixMarket matrix coordinate real {kind}\n{n} {n} {k}\n{body}\n")
+    el = rlc.read_mtx(str(path), True)
+    # main.cxx: read, then symmetrize unless the input is known to be symmetric
+    with ctx.upload_edges(el, not symmetric) as dg:
+        got = dg.csr()
+        r1 = dg.run(rlc.options(mode=rlc.MODE_SYNC))
+    exp = expected_csr(el.n, el.src, el.dst, el.weights, not symmetric)
+    same_graph(got, exp)
+    lab, it, _ = O.lpa_sync(exp)
+    assert r1.iterations == it and np.array_equal(r1.membership, lab)

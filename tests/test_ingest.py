@@ -1,0 +1,126 @@
+"""Matrix Market reader (host): the reference reader's rules (inc/mtx.hxx:30-55, :151-187).
+
+Checked against a line-by-line restatement of those rules in Python on small files, and on a
+larger random file (the parallel parser cuts it at line boundaries).
+"""
+import numpy as np
+import pytest
+
+import rlc
+
+
+def write(tmp_path, name, text):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def reference_parse(text, weighted):
+    """The reference reader, line by line: header, then (u, v[, w]) with w = w or 1; symmetric files
+    list (v, u) right after (u, v); ids shifted to 0-based."""
+    lines = text.splitlines()
+    i, sym = 0, False
+    while lines[i].startswith("%"):
+        if lines[i].startswith("%%"):
+            h = lines[i].split()
+            sym = len(h) > 4 and h[4] in ("symmetric", "skew-symmetric")
+        i += 1
+    rows, cols, _ = map(int, lines[i].split())
+    src, dst, w = [], [], []
+    for ln in lines[i + 1:]:
+        t = ln.split()
+        if not t or t[0].startswith("%"):
+            continue
+        u, v = int(t[0]) - 1, int(t[1]) - 1
+        x = float(t[2]) if weighted and len(t) > 2 else 0.0
+        x = x if x != 0 else 1.0
+        src.append(u), dst.append(v), w.append(x)
+        if sym:
+            src.append(v), dst.append(u), w.append(x)
+    return max(rows, cols), np.array(src, np.uint32), np.array(dst, np.uint32), np.array(w, np.float32), sym
+
+
+def check(path, text, weighted):
+    el = rlc.read_mtx(path, weighted)
+    n, s, d, w, sym = reference_parse(text, weighted)
+    assert el.n == n and el.symmetric == sym
+    assert np.array_equal(el.src, s) and np.array_equal(el.dst, d)
+    if weighted:
+        assert np.array_equal(el.weights, w)
+    else:
+        assert el.weights is None
+
+
+GENERAL = """%%MatrixMarket matrix coordinate real general
+% a comment
+%
+5 5 6
+1 2 0.5
+2 3 0
+3 1 2.25
+4 4 1.0
+5 1 -0
+2 1 7
+"""
+
+SYMMETRIC = """%%MatrixMarket matrix coordinate pattern symmetric
+4 4 3
+2 1
+3 2
+4 4
+"""
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_general_file(tmp_path, weighted):
+    check(write(tmp_path, "g.mtx", GENERAL), GENERAL, weighted)
+
+
+def test_general_values(tmp_path):
+    el = rlc.read_mtx(write(tmp_path, "g.mtx", GENERAL), True)
+    assert el.n == 5 and not el.symmetric
+    assert el.src.tolist() == [0, 1, 2, 3, 4, 1] and el.dst.tolist() == [1, 2, 0, 3, 0, 0]
+    assert el.weights.tolist() == [0.5, 1.0, 2.25, 1.0, 1.0, 7.0]  # 0 and -0 become 1
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_symmetric_file(tmp_path, weighted):
+    check(write(tmp_path, "s.mtx", SYMMETRIC), SYMMETRIC, weighted)
+    el = rlc.read_mtx(str(tmp_path / "s.mtx"), weighted)
+    assert el.symmetric and el.src.tolist() == [1, 0, 2, 1, 3, 3] and el.dst.tolist() == [0, 1, 1, 2, 3, 3]
+
+
+def test_rectangular_and_crlf(tmp_path):
+    text = "%%MatrixMarket matrix coordinate integer general\r\n3 7 2\r\n1 7 3\r\n3 2 4\r\n"
+    el = rlc.read_mtx(write(tmp_path, "r.mtx", text), True)
+    assert el.n == 7 and el.src.tolist() == [0, 2] and el.dst.tolist() == [6, 1]
+    assert el.weights.tolist() == [3.0, 4.0]
+
+
+def test_large_random_file(tmp_path):
+    r = np.random.default_rng(3)
+    n, k = 5000, 40000
+    u, v = r.integers(1, n + 1, k), r.integers(1, n + 1, k)
+    w = np.round(r.uniform(0, 3, k), 3)
+    body = "\n".join(f"{a} {b} {c}" for a, b, c in zip(u, v, w))
+    text = f"%%MatrixMarket matrix coordinate real general\n{n} {n} {k}\n{body}\n"
+    check(write(tmp_path, "big.mtx", text), text, True)
+
+
+@pytest.mark.parametrize("text", [
+    "5 5 1\n1 2\n",                                                      # no banner
+    "%%MatrixMarket matrix array real general\n2 2\n1\n",                # not coordinate
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n1 4 1.0\n",   # id beyond n
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n0 1 1.0\n",   # id 0 (1-based format)
+    "%%MatrixMarket matrix coordinate real general\n3 3 1\n1 x 1.0\n",   # malformed
+])
+def test_errors(tmp_path, text):
+    with pytest.raises(rlc.RlcError) as e:
+        rlc.read_mtx(write(tmp_path, "bad.mtx", text), True)
+    assert e.value.status == 2
+
+
+def test_missing_file(tmp_path):
+    with pytest.raises(rlc.RlcError) as e:
+        rlc.read_mtx(str(tmp_path / "none.mtx"))
+    assert e.value.status == 1
