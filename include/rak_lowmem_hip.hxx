@@ -20,6 +20,7 @@
 //   - a run may be deterministic (RLC_MODE_SYNC) through the rlc_options overloads.
 #pragma once
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -141,6 +142,20 @@ inline rlc_options options(const RakOptions &o, int slots) {
 }
 
 }  // namespace rlc_hip
+
+/** The reference's result line (main.cxx:63-71), which its log tooling parses (process.js:8):
+ *  "{TTT threads} -> {time ms, mark ms, init ms, memory GB, iters, modularity, communities} technique". */
+template <class K>
+inline std::string formatResult(const RakResult<K> &r, double modularity, size_t communities, const char *technique,
+                                int threads) {
+  char buf[512];
+  std::snprintf(buf, sizeof(buf),
+                "{%03d threads} -> {%09.1fms, %09.1fms mark, %09.1fms init, %09.4fGB memory, %04d iters, %01.9f "
+                "modularity, %zu communities} %s",
+                threads, r.time, r.markingTime, r.initializationTime, r.memory, r.iterations, modularity, communities,
+                technique);
+  return buf;
+}
 
 /** rakLowmemStaticCuda<SLOTS>(x, o): SLOTS=8 Misra-Gries (νMG8-LPA), SLOTS=1 Boyer-Moore (νBM-LPA). */
 template <int SLOTS = 8, bool RESCAN = false, bool TRYWARP = true, bool TRYMERGE = true, class G>

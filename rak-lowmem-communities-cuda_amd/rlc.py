@@ -611,6 +611,13 @@ class DeviceGraph:
         return q.value, k.value
 
 
+def format_result(r: RakResult, modularity: float, communities: int, technique: str, threads: int = 0) -> str:
+    """The reference's result line (main.cxx:63-71), the line its log tooling parses."""
+    return ("{%03d threads} -> {%09.1fms, %09.1fms mark, %09.1fms init, %09.4fGB memory, %04d iters, "
+            "%01.9f modularity, %d communities} %s" % (threads, r.time, r.markingTime, r.initializationTime, r.memory,
+                                                       r.iterations, modularity, communities, technique))
+
+
 def iteration_stats(st: RunStats):
     L = min(st.iterations, MAX_ITER_STATS)
     return np.array([[st.active[i], st.edges[i], st.changed[i], st.changed_edges[i]] for i in range(L)],

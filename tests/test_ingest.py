@@ -124,3 +124,21 @@ def test_missing_file(tmp_path):
     with pytest.raises(rlc.RlcError) as e:
         rlc.read_mtx(str(tmp_path / "none.mtx"))
     assert e.value.status == 1
+
+
+# The result-line pattern of the reference's log tooling (process.js:8), restated: a run printed
+# with rlc.format_result / formatResult (include/rak_lowmem_hip.hxx) must be read back by it.
+RESULT_LINE = (r"^\{(.+?) threads\} -> \{(.+?)ms, (.+?)ms mark, (.+?)ms init, (.+?)GB memory, (.+?) iters, "
+               r"(.+?) modularity, (.+?) communities\} (.+)")
+
+
+def test_result_line_format():
+    import re
+    st = rlc.RunStats()
+    r = rlc.RakResult(np.zeros(3, np.uint32), 7, 12.25, 0.5, 0.125, 0.0313, st)
+    line = rlc.format_result(r, 0.731234567, 4096, "rakLowmemStaticCudaMg", threads=64)
+    assert line == ("{064 threads} -> {0000012.2ms, 0000000.5ms mark, 0000000.1ms init, 0000.0313GB memory, "
+                    "0007 iters, 0.731234567 modularity, 4096 communities} rakLowmemStaticCudaMg")
+    m = re.match(RESULT_LINE, line, re.M)
+    assert m and [float(x) for x in m.groups()[:8]] == [64, 12.2, 0.5, 0.1, 0.0313, 7, 0.731234567, 4096]
+    assert m.group(9) == "rakLowmemStaticCudaMg"
