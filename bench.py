@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "rak-lowmem-communities-cuda_amd"))
 
 HBM_PEAK_GBS = 8000.0
+METHOD = {8: "MG8", 1: "BM", 0: "exact hashtable vote (rakStaticCuda)"}
 
 
 def log(*a):
@@ -374,7 +375,7 @@ def main():
     achieved = total_bytes / (move_ms * 1e-3) / 1e9 if move_ms > 0 else 0.0
     value = total_edges / wall
     line = {
-        "metric": "traversed edges/s per LPA iter (MG8)",
+        "metric": f"traversed edges/s per LPA iter ({'MG8' if args.slots == 8 else METHOD[args.slots]})",
         "value": value,
         "unit": "edges/s",
         "n_gpus": 1,
@@ -387,7 +388,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{name}, {'MG8' if args.slots == 8 else 'BM'} async, tol 0.05, max 20 iterations",
+            "workload": f"{name}, {METHOD[args.slots]} async, tol 0.05, max 20 iterations",
             "n": g.n, "m": g.m, "iterations": iters, "modularity": q, "communities": ncom,
         },
         "roofline": {

@@ -5,6 +5,7 @@
 //   RakOptions                          inc/rak.hxx:19-40
 //   RakResult<K>                        inc/rak.hxx:53-97
 //   rakLowmemStaticCuda<SLOTS,...>(x,o) inc/rakLowmemCuda.hxx:624-632
+//   rakStaticCuda(x, o)                 inc/rakCuda.hxx:438-446 (the exact hashtable vote)
 //   rakLowmemCuda(x, init, o)           rakLowmemInvokeCuda with an initial membership
 //                                       (inc/rakLowmemCuda.hxx:538-607, its `fi` hook at :590)
 //   G                                   the reference's graph concept (inc/Graph.hxx:59-168): key_type,
@@ -162,6 +163,15 @@ template <int SLOTS = 8, bool RESCAN = false, bool TRYWARP = true, bool TRYMERGE
 inline auto rakLowmemStaticCuda(const G &x, const RakOptions &o = {}) {
   static_assert(SLOTS == 1 || SLOTS == 8, "SLOTS must be 1 (Boyer-Moore) or 8 (Misra-Gries)");
   return rlc_hip::run(x, nullptr, rlc_hip::options(o, SLOTS));
+}
+
+/** rakStaticCuda(x, o) (inc/rakCuda.hxx:438-446): ν-LPA, the exact weighted vote in a per-vertex
+ *  hashtable, pick-less every 4 iterations (rakLoopCuU, inc/rakCuda.hxx:289-304). */
+template <class G>
+inline auto rakStaticCuda(const G &x, const RakOptions &o = {}) {
+  rlc_options r = rlc_hip::options(o, RLC_SKETCH_EXACT);
+  r.pickstep = 4;
+  return rlc_hip::run(x, nullptr, r);
 }
 
 /** Same, with full control of the library options (mode, sweeps, thresholds, timing). */

@@ -57,8 +57,10 @@ enum {
   RLC_MODE_SYNC = 1   /* deterministic: Jacobi label snapshot, changes applied after each iteration */
 };
 
-/* Sketch: rakLowmemStaticCuda<SLOTS>: SLOTS=1 Boyer-Moore, SLOTS=8 Misra-Gries. */
-enum { RLC_SKETCH_BM = 1, RLC_SKETCH_MG8 = 8 };
+/* Vote: rakLowmemStaticCuda<SLOTS>: SLOTS=1 Boyer-Moore, SLOTS=8 Misra-Gries; and the reference's
+ * exact method rakStaticCuda (ν-LPA, inc/rakCuda.hxx:122-446): every label's exact weighted sum in
+ * a per-vertex hashtable of 2 entries per edge (16 bytes per edge of working memory). */
+enum { RLC_SKETCH_EXACT = 0, RLC_SKETCH_BM = 1, RLC_SKETCH_MG8 = 8 };
 
 /* Timing of the iteration loop. */
 enum {
@@ -72,7 +74,7 @@ typedef struct rlc_options {
   int32_t repeat;         /* RakOptions::repeat [1] */
   double tolerance;       /* RakOptions::tolerance [0.05]: stop when changed/N <= tolerance */
   int32_t max_iterations; /* RakOptions::maxIterations [20] */
-  int32_t slots;          /* RLC_SKETCH_BM or RLC_SKETCH_MG8 [8] */
+  int32_t slots;          /* RLC_SKETCH_MG8, RLC_SKETCH_BM or RLC_SKETCH_EXACT [8] */
   int32_t mode;           /* RLC_MODE_ASYNC or RLC_MODE_SYNC [ASYNC] */
   int32_t pickstep;       /* iterations l with l % pickstep == 0 only move to smaller labels and skip the
                              convergence test; 0 disables [8] */

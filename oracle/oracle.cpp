@@ -13,7 +13,10 @@
 //                       counterpart: the reference updates labels in place). Per vertex it applies
 //                       the reference's own sketch steps (rakLowmem.hxx:63-89, :115-128, :135-172)
 //                       to a label snapshot with a fresh sketch; vertices of degree >= merge_degree
-//                       use the P-part strided scan merged by a fixed binary tree.
+//                       use the P-part strided scan merged by a fixed binary tree. slots = 0: the
+//                       exact hashtable vote of rakStaticCuda's thread kernel (rakCuda.hxx:123-174,
+//                       hashtableCuda.hxx:28-123), pinned by the exact-maximum and slot-order tests
+//                       of tests/test_oracle.py (the CUDA reference cannot run here).
 //   orc_modularity      modularityBy / modularityCommunities / edgeWeight  inc/properties.hxx:67-192
 //   orc_communities     communities(x, vcom).size()                        inc/properties.hxx:352-364
 //   orc_sketch_choose   one vertex's choice from an ordered (label, weight) list, for unit tests.
@@ -145,6 +148,50 @@ void choose_bm(uint64_t deg, int parts, Get get, uint32_t &cbest, float &wbest) 
   wbest = mw[0];
 }
 
+// rakStaticCuda's vote (ν-LPA), as its thread-per-vertex kernel computes it (rakCuda.hxx:123-174):
+// the exact weighted label sums in an open-addressing table of H = nextPow2(deg) - 1 slots, keys
+// label + 1, probed with hashtable type 3 (hashtableCuda.hxx:58-70: i += di, di = 2 di + k % T,
+// T = nextPow2(H) - 1, 64-bit unsigned arithmetic); a key that finds no slot within H probes is
+// dropped. nextPow2Cud(x) = 1 << (32 - clz32(x)) (_cuda.hxx:491-493). The choice is slot 0 after a
+// scan that moves every strictly heavier slot there (hashtableMaxThreadCudU, hashtableCuda.hxx:
+// 116-123); found = its key is non-zero.
+inline uint64_t next_pow2_cud(uint64_t x) {
+  const uint32_t v = (uint32_t)x;
+  const int clz = v ? __builtin_clz(v) : 32;
+  return 1ull << (32 - clz);
+}
+
+template <class Get>
+void choose_ht(uint64_t deg, Get get, bool &found, uint32_t &cbest, float &wbest) {
+  const uint64_t H = next_pow2_cud(deg) - 1, T = next_pow2_cud(H) - 1;
+  std::vector<uint32_t> hk(H, 0);
+  std::vector<float> hv(H, 0.0f);
+  uint32_t c;
+  float w;
+  for (uint64_t e = 0; e < deg; ++e) {
+    if (!get(e, c, w)) continue;
+    const uint32_t k = c + 1;
+    uint64_t i = k, di = 1;
+    for (uint64_t t = 0; t < H; ++t) {
+      const uint64_t j = i % H;
+      if (hk[j] == k || hk[j] == 0) {
+        hk[j] = k;
+        hv[j] += w;
+        break;
+      }
+      i += di;
+      di = di * 2 + (k % T);
+    }
+  }
+  uint32_t k0 = H ? hk[0] : 0;
+  float w0 = H ? hv[0] : 0.0f;
+  for (uint64_t j = 1; j < H; ++j)
+    if (hv[j] > w0) { w0 = hv[j]; k0 = hk[j]; }
+  found = k0 != 0;
+  cbest = k0 - 1;
+  wbest = w0;
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -156,12 +203,20 @@ extern "C" {
 int orc_abi_version() { return 1; }
 
 // One vertex's choice in the synchronous mode, from an ordered list of (label, weight) pairs
-// (self-loops already removed). slots: 8 (MG) or 1 (BM). parts: 1 = sequential, else P.
+// (self-loops already removed). slots: 8 (MG), 1 (BM) or 0 (exact hashtable vote: c_out = label,
+// or 0xFFFFFFFF when none is found). parts: 1 = sequential, else P (sketches only).
 void orc_sketch_choose(const uint32_t *labels, const float *weights, uint64_t k, int slots, int parts,
                        uint32_t *c_out, float *w_out) {
   auto get = [&](uint64_t i, uint32_t &c, float &w) { c = labels[i]; w = weights[i]; return true; };
-  if (slots == 1) choose_bm(k, parts, get, *c_out, *w_out);
-  else choose_mg(k, parts, get, *c_out, *w_out);
+  if (slots == 0) {
+    bool found;
+    choose_ht(k, get, found, *c_out, *w_out);
+    if (!found) *c_out = EMPTY;
+  } else if (slots == 1) {
+    choose_bm(k, parts, get, *c_out, *w_out);
+  } else {
+    choose_mg(k, parts, get, *c_out, *w_out);
+  }
 }
 
 // Deterministic synchronous LPA (this library's RLC_MODE_SYNC). Returns the iteration count.
@@ -199,9 +254,15 @@ int orc_lpa_sync(uint64_t n, const uint64_t *off, const uint32_t *edg, const flo
       const int P = (merge_degree > 0 && (int64_t)deg >= merge_degree) ? merge_parts : 1;
       uint32_t c;
       float w;
-      if (slots == 1) choose_bm(deg, P, get, c, w);
-      else choose_mg(deg, P, get, c, w);
-      if (w > 0 && c != d && !(pickless && c > d)) mv[u] = c;
+      bool found;
+      if (slots == 0) {
+        choose_ht(deg, get, found, c, w);
+      } else {
+        if (slots == 1) choose_bm(deg, P, get, c, w);
+        else choose_mg(deg, P, get, c, w);
+        found = w > 0;
+      }
+      if (found && c != d && !(pickless && c > d)) mv[u] = c;
     }
     uint64_t C = 0, X = 0;
     std::fill(aff.begin(), aff.end(), 0);

@@ -66,6 +66,8 @@ struct rlc_graph {
   uint8_t *d_aff = nullptr;
   uint64_t *d_moves = nullptr;
   uint32_t *d_lbuf = nullptr;
+  uint32_t *d_htk = nullptr;  // exact vote: hashtable keys / weights, 2 entries per edge position
+  float *d_htw = nullptr;
   DevCounters *d_ctr = nullptr;
   uint32_t *d_init = nullptr;
   double *d_ctot = nullptr;
@@ -89,7 +91,7 @@ struct rlc_graph {
   ChunkPlan plan{};
 
   DevGraph dev() const { return DevGraph{d_off, d_edg, unit ? nullptr : d_wei, n, m}; }
-  DevState state() const { return DevState{d_lab, d_lbuf, d_aff, d_moves, d_ctr}; }
+  DevState state() const { return DevState{d_lab, d_lbuf, d_aff, d_moves, d_ctr, d_htk, d_htw}; }
 };
 
 namespace {
@@ -110,7 +112,7 @@ void free_graph(rlc_graph *g) {
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->hexec) (void)hipGraphExecDestroy(g->hexec);
   if (g->hgraph) (void)hipGraphDestroy(g->hgraph);
-  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_lbuf, g->d_aff, g->d_moves,
+  void *ptrs[] = {g->d_off, g->d_edg, g->d_wei, g->d_perm, g->d_lab, g->d_lbuf, g->d_htk, g->d_htw, g->d_aff, g->d_moves,
                   g->d_ctr, g->d_init, g->d_ctot, g->d_seen, g->d_modout, g->d_plan,
                   g->d_K, g->d_mwin, g->d_mstart};
   for (void *p : ptrs)
@@ -215,6 +217,25 @@ Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
   *st = RLC_OK;
   Schedule sc;
   const size_t S = g->h_seg.size() - 1;
+  if (o->slots == RLC_SKETCH_EXACT) {
+    // exact vote: SYNC one lane per vertex (deterministic sums); ASYNC a wavefront per vertex of
+    // degree >= 32 (the reference's block kernel threshold, inc/rakCuda.hxx:318) and a lane per
+    // vertex below, segment after segment
+    const uint64_t D = o->mode == RLC_MODE_SYNC ? ~0ull : 32;
+    const bool one_sweep = o->sweeps == 1 || o->mode == RLC_MODE_SYNC;
+    for (size_t s = 0; s < (one_sweep ? 1 : S); ++s) {
+      MoveLaunch wave{64, empty_ranges()}, lane{1, empty_ranges()};
+      for (size_t q = one_sweep ? 0 : s; q < (one_sweep ? S : s + 1); ++q) {
+        const uint64_t sb = g->h_seg[q], se = g->h_seg[q + 1];
+        const uint64_t cut = D == ~0ull ? sb : first_below(g, sb, se, D);
+        add_range(wave.ranges, sb, cut, false);
+        add_range(lane.ranges, cut, se, true);
+      }
+      sc.launches.push_back(wave);
+      sc.launches.push_back(lane);
+    }
+    return sc;
+  }
   if (o->mode == RLC_MODE_SYNC) {
     const uint64_t D = o->merge_degree > 0 ? (uint64_t)o->merge_degree : ~0ull;
     MoveLaunch grp{o->merge_parts, empty_ranges()}, lane{1, empty_ranges()};
@@ -270,8 +291,11 @@ rlc_status rlc::check_options(const rlc_options *o) {
   if (o->repeat < 1) return fail(RLC_ERR_INVALID_ARGUMENT, "options.repeat must be >= 1");
   if (o->max_iterations < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.max_iterations must be >= 0");
   if (!(o->tolerance >= 0)) return fail(RLC_ERR_INVALID_ARGUMENT, "options.tolerance must be >= 0");
-  if (o->slots != RLC_SKETCH_BM && o->slots != RLC_SKETCH_MG8)
-    return fail(RLC_ERR_INVALID_ARGUMENT, "options.slots must be 1 (Boyer-Moore) or 8 (Misra-Gries)");
+  if (o->slots != RLC_SKETCH_EXACT && o->slots != RLC_SKETCH_BM && o->slots != RLC_SKETCH_MG8)
+    return fail(RLC_ERR_INVALID_ARGUMENT,
+                "options.slots must be 8 (Misra-Gries), 1 (Boyer-Moore) or 0 (exact hashtable vote)");
+  if (o->slots == RLC_SKETCH_EXACT && o->mode == RLC_MODE_SYNC && o->merge_degree > 0)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_degree is for the sketches, not the exact vote");
   if (o->mode != RLC_MODE_ASYNC && o->mode != RLC_MODE_SYNC)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.mode must be RLC_MODE_ASYNC or RLC_MODE_SYNC");
   if (o->pickstep < 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.pickstep must be >= 0");
@@ -581,6 +605,10 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   }
   const bool sync = o->mode == RLC_MODE_SYNC;
   if (sync && !g->d_moves && (st = dev_alloc(&g->d_moves, n)) != RLC_OK) return st;
+  if (o->slots == RLC_SKETCH_EXACT && !g->d_htk) {
+    if ((st = dev_alloc(&g->d_htk, 2 * g->m)) != RLC_OK || (st = dev_alloc(&g->d_htw, 2 * g->m)) != RLC_OK) return st;
+    g->key.valid = false;
+  }
   if (o->slots == RLC_SKETCH_MG8 && !o->no_rescan && !g->d_lbuf) {
     if ((st = dev_alloc(&g->d_lbuf, g->m)) != RLC_OK) return st;
     g->key.valid = false;  // captured launches embed the state pointers
@@ -591,13 +619,14 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   const DevGraph dg = g->dev();
   const DevState ds = g->state();
   LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, !o->no_rescan, o->pickstep, o->tolerance, o->max_iterations};
+  lp.exact = o->slots == RLC_SKETCH_EXACT;
   const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
 
   if (graph_mode && o->max_iterations > 0) {
     ExecKey k;
     k.valid = true;
     k.sync = sync;
-    k.mg = lp.mg;
+    k.mg = o->slots;
     k.pickstep = o->pickstep;
     k.max_iterations = o->max_iterations;
     k.tolerance = o->tolerance;
@@ -673,7 +702,8 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     stats->init_ms = t_init / o->repeat;
     stats->marking_ms = t_mark / o->repeat;
     const uint64_t state_bytes = n * (4 + 1) + (sync ? n * 8 : 0) + sizeof(DevCounters) + (init_labels ? n * 4 : 0) +
-                                 (o->slots == RLC_SKETCH_MG8 && !o->no_rescan ? g->m * 4 : 0);
+                                 (o->slots == RLC_SKETCH_MG8 && !o->no_rescan ? g->m * 4 : 0) +
+                                 (o->slots == RLC_SKETCH_EXACT ? g->m * 16 : 0);
     stats->memory_gb = (float)((double)state_bytes / (1024.0 * 1024.0 * 1024.0));
     stats->changed_last = (uint32_t)hc.last_changed;
     const uint64_t eb = g->unit ? 8 : 12;

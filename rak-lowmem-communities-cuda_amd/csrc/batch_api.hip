@@ -62,6 +62,8 @@ rlc_status rlc::check_batch_options(const rlc_options *o) {
   rlc_status st = check_options(o);
   if (st != RLC_OK) return st;
   if (o->no_rescan) return fail(RLC_ERR_INVALID_ARGUMENT, "options.no_rescan is not supported by batches");
+  if (o->slots == RLC_SKETCH_EXACT)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "the exact hashtable vote runs on single graphs (rlc_run), not in batches");
   if (o->mode == RLC_MODE_SYNC && o->merge_degree > 0 && o->merge_parts == 256)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.merge_parts = 256 is not supported by batches (4, 16 or 64)");
   return RLC_OK;

@@ -314,6 +314,215 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
   flush_stats(s.ctr, sa, se, sc, sx);
 }
 
+// ---- the exact vote of rakStaticCuda (ν-LPA) ----------------------------------------------------
+// Every label's exact weighted sum in an open-addressing table of H = nextPow2(deg) - 1 slots
+// (keys label + 1, 0 = empty) at positions 2 off[u] .. of the 2m-entry buffers, probed as the
+// reference's hashtable type 3 (inc/hashtableCuda.hxx:58-70: i += di, di = 2 di + k % T, 64-bit);
+// a key that finds no slot within H probes is dropped. The choice is the first slot of strictly
+// largest weight (inc/hashtableCuda.hxx:116-123); "found" = its key is non-zero
+// (inc/rakCuda.hxx:150-155). nextPow2(x) = 1 << (32 - clz32(x)) (inc/_cuda.hxx:491-493).
+__device__ __forceinline__ uint64_t next_pow2_ht(uint64_t x) {
+  const uint32_t v = (uint32_t)x;
+  return 1ull << (32 - (v ? __clz(v) : 32));
+}
+
+constexpr int kHtLdsSlots = 1023;  // wavefront tables of at most this many slots live in LDS
+
+// One lane per vertex, sums in CSR order (the thread kernel, inc/rakCuda.hxx:123-174).
+template <bool UNIT>
+__device__ __forceinline__ void ht_choose_lane(const DevGraph &g, const uint32_t *lab, uint32_t u, uint64_t b, uint64_t e,
+                                               uint32_t *hk, float *hv, bool &found, uint32_t &cb) {
+  const uint64_t H = next_pow2_ht(e - b) - 1, T = next_pow2_ht(H) - 1;
+  for (uint64_t j = 0; j < H; ++j) {
+    hk[j] = 0;
+    hv[j] = 0.0f;
+  }
+  for_edges<UNIT, 1>(g, lab, u, b, e, [&](uint32_t c, float w) {
+    const uint32_t k = c + 1;
+    uint64_t i = k, di = 1;
+    for (uint64_t t = 0; t < H; ++t) {
+      const uint64_t j = i % H;
+      const uint32_t h = hk[j];
+      if (h == k || h == 0) {
+        hk[j] = k;
+        hv[j] += w;
+        break;
+      }
+      i += di;
+      di = di * 2 + (k % T);
+    }
+  });
+  uint32_t k0 = H ? hk[0] : 0u;
+  float w0 = H ? hv[0] : 0.0f;
+  for (uint64_t j = 1; j < H; ++j) {
+    const float x = hv[j];
+    if (x > w0) {
+      w0 = x;
+      k0 = hk[j];
+    }
+  }
+  found = k0 != 0;
+  cb = k0 - 1;
+}
+
+template <bool SYNC, bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_ht_lane(DevGraph g, DevState s, RangeSet rs, int pickstep) {
+  if (s.ctr->done) return;
+  const uint32_t iter = s.ctr->iter;
+  const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
+  unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
+  __shared__ unsigned int wcount[kBlock / 64];
+  __shared__ unsigned long long bbase;
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  constexpr uint64_t WPB = kBlock / 64;
+  const uint64_t nch = rs.acc[rs.count];
+  for (uint64_t tb = (uint64_t)blockIdx.x * WPB; tb < nch; tb += (uint64_t)gridDim.x * WPB) {
+    const uint64_t t = tb + (uint64_t)wave;
+    bool valid = false;
+    uint32_t u = 0;
+    if (t < nch) {
+      const uint32_t r = range_of(rs, t);
+      const uint64_t p = rs.first[r] + (t - rs.acc[r]) * 64 + (uint64_t)lane;
+      valid = p < rs.last[r];
+      u = valid ? (uint32_t)p : 0u;
+    }
+    const bool act = valid && s.aff[u] != 0;
+    uint64_t deg = 0;
+    bool moved = false;
+    uint32_t cb = 0;
+    if (act) {
+      const uint64_t b = g.off[u], e = g.off[(uint64_t)u + 1];
+      deg = e - b;
+      const uint32_t d = s.lab[u];
+      bool found = false;
+      ht_choose_lane<UNIT>(g, s.lab, u, b, e, s.htk + 2 * b, s.htw + 2 * b, found, cb);
+      moved = found && cb != d && !(pickless && cb > d);
+      s.aff[u] = 0;
+      if (!SYNC && moved) {
+        s.lab[u] = cb;
+        mark_neighbours(g, s.aff, u, b, e, 1);
+      }
+    }
+    sa += act ? 1 : 0;
+    se += deg;
+    if (SYNC) {
+      const unsigned long long mask = __ballot(moved);
+      if (lane == 0) wcount[wave] = (unsigned int)__popcll(mask);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        unsigned int tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wcount[w];
+        bbase = tot ? atomicAdd(&s.ctr->nmoves, (unsigned long long)tot) : 0ull;
+      }
+      __syncthreads();
+      if (moved) {
+        unsigned long long pos = bbase;
+        for (int w = 0; w < wave; ++w) pos += wcount[w];
+        pos += lanes_below(mask);
+        s.moves[pos] = ((uint64_t)u << 32) | cb;
+      }
+      __syncthreads();
+    } else {
+      sc += moved ? 1 : 0;
+      sx += moved ? deg : 0;
+    }
+  }
+  flush_stats(s.ctr, sa, se, sc, sx);
+}
+
+// Asynchronous mode, degree >= 32: one wavefront per vertex (the block kernel,
+// inc/rakCuda.hxx:201-257). Lanes take edges i, i+64, .. and insert with atomics (compare-and-swap
+// on the key, add on the weight), in LDS when the table has at most kHtLdsSlots slots, else in the
+// vertex's part of the global buffers. The choice: each lane's first strictly heaviest slot over
+// slots lane, lane+64, .., then the heaviest over the lanes, the lower slot on ties. Table entries
+// are read with device-scope atomic loads: other lanes' atomics update them behind the L1.
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_ht_wave(DevGraph g, DevState s, RangeSet rs, int pickstep) {
+  if (s.ctr->done) return;
+  const uint32_t iter = s.ctr->iter;
+  const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
+  __shared__ uint32_t lk[kBlock / 64][kHtLdsSlots];
+  __shared__ float lv[kBlock / 64][kHtLdsSlots];
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
+  const uint64_t total = rs.acc[rs.count];
+  const uint64_t nw = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t idx = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6; idx < total; idx += nw) {
+    const uint32_t r = range_of(rs, idx);
+    const uint32_t u = (uint32_t)(rs.first[r] + (idx - rs.acc[r]));
+    if (s.aff[u] == 0) continue;  // wave-uniform
+    const uint64_t b = g.off[u], e = g.off[(uint64_t)u + 1], deg = e - b;
+    const uint32_t d = s.lab[u];
+    const uint64_t H = next_pow2_ht(deg) - 1, T = next_pow2_ht(H) - 1;
+    const bool in_lds = H <= (uint64_t)kHtLdsSlots;
+    uint32_t *hk = in_lds ? lk[wave] : s.htk + 2 * b;
+    float *hv = in_lds ? lv[wave] : s.htw + 2 * b;
+    for (uint64_t j = (uint64_t)lane; j < H; j += 64) {
+      hk[j] = 0;
+      hv[j] = 0.0f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for_edges<UNIT, 64>(g, s.lab, u, b + (uint64_t)lane, e, [&](uint32_t c, float w) {
+      const uint32_t k = c + 1;
+      uint64_t i = k, di = 1;
+      for (uint64_t t = 0; t < H; ++t) {
+        const uint64_t j = i % H;
+        const uint32_t h = __hip_atomic_load(&hk[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h == k || h == 0) {
+          const uint32_t old = h == k ? k : atomicCAS(&hk[j], 0u, k);
+          if (old == 0u || old == k) {
+            atomicAdd(&hv[j], w);
+            break;
+          }
+        }
+        i += di;
+        di = di * 2 + (k % T);
+      }
+    });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    float bw = -1.0f;
+    uint64_t bj = ~0ull;
+    for (uint64_t j = (uint64_t)lane; j < H; j += 64) {
+      const float x = __hip_atomic_load(&hv[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (x > bw) {
+        bw = x;
+        bj = j;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const float ow = __shfl_xor(bw, o);
+      const uint64_t oj = __shfl_xor(bj, o);
+      if (ow > bw || (ow == bw && oj < bj)) {
+        bw = ow;
+        bj = oj;
+      }
+    }
+    const uint32_t k0 = bj < H ? __hip_atomic_load(&hk[bj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+    const uint32_t cb = k0 - 1;
+    const bool moved = k0 != 0 && cb != d && !(pickless && cb > d);
+    if (lane == 0) {
+      s.aff[u] = 0;
+      sa += 1;
+      se += deg;
+    }
+    if (moved) {
+      if (lane == 0) {
+        s.lab[u] = cb;
+        sc += 1;
+        sx += deg;
+      }
+      mark_neighbours(g, s.aff, u, b + (uint64_t)lane, e, 64);
+    }
+    __builtin_amdgcn_wave_barrier();  // the LDS table is reused by the wave's next vertex
+  }
+  flush_stats(s.ctr, sa, se, sc, sx);
+}
+
 // ---- asynchronous mode: vertices cut into chunks, one wavefront per chunk -------------------
 // Five launches, with kernel boundaries as the only synchronisation (on this GPU a device-scope
 // fence writes back or invalidates the whole L2 of an XCD, far more costly than a launch):
@@ -738,10 +947,30 @@ static void launch_moves_w(const DevGraph &g, const DevState &s, const Schedule 
   }
 }
 
+template <bool SYNC, bool UNIT>
+static void launch_ht(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p, hipStream_t st) {
+  for (const MoveLaunch &L : sc.launches) {
+    const uint64_t items = L.ranges.acc[L.ranges.count];
+    if (items == 0) continue;
+    if (L.lanes == 1)
+      k_ht_lane<SYNC, UNIT><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges, p.pickstep);
+    else
+      k_ht_wave<UNIT><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges, p.pickstep);
+  }
+}
+
 void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
                       hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1) {
   if (ev_move0) (void)hipEventRecord(ev_move0, st);
-  if (p.sync) {
+  if (p.exact) {
+    if (p.sync) {
+      if (g.wei) launch_ht<true, false>(g, s, cl, p, st);
+      else launch_ht<true, true>(g, s, cl, p, st);
+    } else {
+      if (g.wei) launch_ht<false, false>(g, s, cl, p, st);
+      else launch_ht<false, true>(g, s, cl, p, st);
+    }
+  } else if (p.sync) {
     if (p.mg) launch_moves_w<true, true>(g, s, cl, p, st);
     else launch_moves_w<true, false>(g, s, cl, p, st);
   } else {

@@ -35,6 +35,8 @@ struct DevState {
   uint8_t *aff;      // vertex-pruning flags: 1 = process in the next iteration
   uint64_t *moves;   // synchronous mode change list: (u << 32) | new label
   DevCounters *ctr;
+  uint32_t *htk = nullptr;  // exact vote: hashtable keys (label + 1), 2 entries per edge position
+  float *htw = nullptr;     // exact vote: hashtable weights
 };
 
 // A set of position ranges [first[r], last[r]) of the degree-descending layout; acc[] is the
@@ -90,6 +92,7 @@ struct LoopParams {
   int pickstep;
   double tolerance;
   int max_iterations;
+  bool exact = false;  // the exact hashtable vote (rakStaticCuda) instead of a sketch
 };
 
 // flags = 1; counters reset (done = max_iterations <= 0)

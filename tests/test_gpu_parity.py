@@ -237,3 +237,32 @@ def test_cpp_dropin_runs():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "ok" in r.stdout
+
+
+# ---- the exact hashtable vote (rakStaticCuda, slots = 0) ----------------------------------------------
+@pytest.mark.parametrize("case", ["cfg1", "weighted", "hubs", "rmat"])
+def test_exact_vote_sync_bit_exact(ctx, case):
+    g = {"cfg1": cfg1, "weighted": lambda: random_weighted(3000, 20000, seed=4),
+         "hubs": lambda: with_hubs(4000, hubs=5, hub_degree=1500, base_degree=10, seed=2),
+         "rmat": lambda: rlc.gen_rmat(14, 16, 0.57, 0.19, 0.19, 0.5, 1.5, seed=3)}[case]()
+    print(f"exact sync {case} n={g.n} m={g.m}", flush=True)
+    with ctx.upload(g) as dg:
+        r = dg.run(rlc.options(mode=rlc.MODE_SYNC, slots=0, pickstep=4))
+    lab, it, ost = O.lpa_sync(g, slots=0, pickstep=4)
+    assert r.iterations == it
+    assert np.array_equal(r.membership, lab)
+    assert np.array_equal(rlc.iteration_stats(r.stats), ost)
+
+
+@pytest.mark.parametrize("case", ["cfg1", "sbm1m", "hubs"])
+def test_exact_vote_async_quality(ctx, case):
+    # asynchronous exact vote against the reference's exact CPU method (GVE-LPA, rakStaticOmp)
+    g = {"cfg1": cfg1, "sbm1m": lambda: rlc.gen_sbm(1 << 20, 1024, 16.0, 0.8, 3),
+         "hubs": lambda: with_hubs(20000, hubs=20, hub_degree=4000, base_degree=16, seed=8, weighted=False)}[case]()
+    with ctx.upload(g) as dg:
+        r = dg.run(rlc.options(slots=0, pickstep=4))
+        q, k = dg.modularity()
+    q_gve = O.modularity(g, O.rak_omp(g, threads=8)[0])
+    print(f"exact async {case}: Q {q:.4f} ({r.iterations} iterations), GVE-LPA {q_gve:.4f}", flush=True)
+    assert 1 <= r.iterations <= 20
+    assert q >= 0.98 * q_gve - 1e-3

@@ -216,3 +216,56 @@ def test_empty_graph_runs_to_the_cap():
     g = rlc.Graph(np.zeros(1, np.uint64), np.zeros(0, np.uint32), None)
     lab, it, st = O.lpa_sync(g, max_iterations=5)
     assert it == 5 and len(lab) == 0
+
+
+# ---- the exact hashtable vote (slots = 0: rakStaticCuda's choice) -----------------------------------
+def exact_sums(labels, weights):
+    d = {}
+    for c, w in zip(labels, weights):
+        d[int(c)] = np.float32(d.get(int(c), np.float32(0)) + np.float32(w))
+    return d
+
+
+@pytest.mark.parametrize("seed", range(30))
+def test_exact_vote_picks_a_maximum(seed):
+    # the chosen label's weight is the exact sum of its weights (in list order), and no label has more
+    r = np.random.default_rng(seed)
+    k = int(r.integers(1, 200))
+    labels = r.integers(0, int(r.integers(1, 40)), k).astype(np.uint32)
+    weights = r.choice([0.5, 1.0, 1.5, 2.0], k).astype(np.float32)
+    c, w = O.sketch_choose(labels, weights, slots=0)
+    sums = exact_sums(labels, weights)
+    assert c in sums and np.float32(w) == sums[c]
+    assert w == max(sums.values())
+
+
+def test_exact_vote_tie_goes_to_first_slot():
+    # with H = nextPow2(deg) - 1 slots and keys label+1 whose residues mod H differ (no collision),
+    # the first slot of maximum weight wins: the smallest (label + 1) % H among tied labels
+    labels = np.array([9, 4, 2, 6, 9, 4, 2, 6], np.uint32)  # deg 8 -> H = 15: residues 10, 5, 3, 7
+    c, w = O.sketch_choose(labels, np.ones(8, np.float32), slots=0)
+    assert (c, w) == (2, 2.0)
+    # deg 6 -> H = T = 7: label 14 -> key 15 -> slot 1; label 0 -> key 1 collides at slot 1, probes
+    # i = 1 + 1 -> slot 2; label 5 -> key 6 -> slot 6. All tied at 2: slot 1 (label 14) wins.
+    labels = np.array([14, 0, 14, 0, 5, 5], np.uint32)
+    c, w = O.sketch_choose(labels, np.ones(6, np.float32), slots=0)
+    assert (c, w) == (14, 2.0)
+
+
+def test_exact_vote_capacity_rule():
+    # H = nextPow2(deg) - 1 with nextPow2(x) = 2^bitlength(x): one neighbour gives one slot (found);
+    # no neighbour gives no slot (nothing found)
+    c, w = O.sketch_choose(np.array([7], np.uint32), np.array([2.0], np.float32), slots=0)
+    assert (c, w) == (7, 2.0)
+    c, w = O.sketch_choose(np.zeros(0, np.uint32), np.zeros(0, np.float32), slots=0)
+    assert c == 0xFFFFFFFF
+
+
+def test_exact_vote_sync_run_quality():
+    # the synchronous exact vote on cliques joined by single edges: one label per clique
+    from graphs import cliques
+    g = cliques([6, 7, 8, 9])
+    lab, it, _ = O.lpa_sync(g, slots=0, pickstep=4)
+    starts = np.cumsum([0, 6, 7, 8])
+    for s, n in zip(starts, [6, 7, 8, 9]):
+        assert len(set(lab[s:s + n].tolist())) == 1
