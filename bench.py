@@ -113,6 +113,16 @@ def bench_batch(args):
     for gid in range(args.graphs):
         lab, it = b.result(gid)
         digest.append((hashlib.blake2b(lab.tobytes(), digest_size=16).hexdigest(), it))
+    if args.batched_only:
+        b.close()
+        ctx.close()
+        print(json.dumps({"metric": "batched graphs/s (batched leg only)", "value": args.graphs * args.steps / wall_b,
+                          "unit": "graphs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": wall_b * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+                          "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                          "config": {"workload": f"{args.graphs} seeded SBM graphs", "batch_steps": st.steps,
+                                     "traversed_edges_per_run": st.traversed_edges}}), flush=True)
+        return
     # reference: each sampled graph alone through rlc_run (its own upload, deterministic mode)
     for gid, g in sample.items():
         with ctx.upload(g) as dg:
@@ -310,6 +320,8 @@ def main():
     ap.add_argument("--batch-seed", type=int, default=1)
     ap.add_argument("--run-sample", type=int, default=256,
                     help="batch workload: graphs timed through rlc_run one at a time")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="batch workload: time the batched leg only (no one-by-one legs, no checks; for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="after timing, run one device-to-device copy of this many bytes (the known-size kernel "
                          "that scales FETCH_SIZE / WRITE_SIZE counters in a rocprofv3 --pmc run)")

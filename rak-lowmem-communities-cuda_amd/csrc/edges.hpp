@@ -191,6 +191,85 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- asynchronous candidate merge of W-lane groups through LDS --------------------------------
+// For the asynchronous mode, a cheaper stand-in for the fixed merge tree (group_merge_tree, kept
+// by the synchronous mode): every lane adds its non-empty slots into its group's open-addressing
+// table in LDS (label -> summed sketch weight; kMergeTable entries per lane, linear probing,
+// float atomic adds in any order); then each lane offers the heaviest entry among its own table
+// positions and the group's 8 heaviest offers become the candidates, heaviest first (ties: the
+// larger label first, so that argmax8<true>, where the highest slot wins ties, prefers the
+// smaller label). All lanes of the group end with the same candidates and summed weights.
+// For a label, the lanes' sketch weights sum to at most its total weight and fall short of it
+// by at most the sum D of the lanes' decrements (per lane, a Misra-Gries weight is within that
+// lane's decrements of the true weight), so certain_argmax(cand, cw, D, ..) keeps its meaning,
+// and a rescan gives exact weights as after the tree. The tree costs log2(W) rounds of 8
+// sequential sketch updates by the whole wave; this costs 8 table inserts per lane and 8
+// group-wide maximum reductions. tk / tw: this wave's kMergeWave entries.
+constexpr int kMergeTable = 16;
+constexpr int kMergeWave = 64 * kMergeTable;
+
+template <int W>
+__device__ __forceinline__ void group_merge_lds(const MG8 &s, uint32_t *tk, float *tw, uint32_t (&cand)[kSlots],
+                                                float (&cw)[kSlots]) {
+  constexpr uint32_t T = (uint32_t)W * kMergeTable;  // entries per group
+  constexpr int LOG2T = __builtin_ctz(T);
+  const int lane = lane_id();
+  const uint32_t gl = (uint32_t)(lane & (W - 1));
+  const uint32_t g0 = (uint32_t)(lane & ~(W - 1)) * kMergeTable;  // group's first entry
+#pragma unroll
+  for (int q = 0; q < kMergeTable; ++q) {
+    tk[g0 + gl + q * W] = kEmpty;
+    tw[g0 + gl + q * W] = 0.0f;
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) {
+    if (s.w[j] > 0.0f) {
+      const uint32_t key = s.c[j];
+      uint32_t h = (key * 0x9E3779B1u) >> (32 - LOG2T);
+      for (uint32_t probe = 0; probe < T; ++probe) {  // at most 8 W keys in T = 16 W entries
+        const uint32_t prev = atomicCAS(&tk[g0 + h], kEmpty, key);
+        if (prev == kEmpty || prev == key) {
+          atomicAdd(&tw[g0 + h], s.w[j]);
+          break;
+        }
+        h = (h + 1) & (T - 1);
+      }
+    }
+  }
+  wave_lds_sync();
+  float bw = 0.0f;
+  uint32_t bk = kEmpty;
+#pragma unroll
+  for (int q = 0; q < kMergeTable; ++q) {
+    const float v = tw[g0 + gl + q * W];
+    const uint32_t k = tk[g0 + gl + q * W];
+    const bool take = v > bw || (v == bw && v > 0.0f && k > bk);
+    bw = take ? v : bw;
+    bk = take ? k : bk;
+  }
+  wave_lds_sync();  // every read done before the table is cleared for the next vertex
+#pragma unroll
+  for (int r = 0; r < kSlots; ++r) {
+    float mw = bw;
+    uint32_t mk = bk;
+#pragma unroll
+    for (int k = 1; k < W; k <<= 1) {
+      const float ow = __shfl_xor(mw, k);
+      const uint32_t ok = __shfl_xor(mk, k);
+      const bool take = ow > mw || (ow == mw && ok > mk);
+      mw = take ? ow : mw;
+      mk = take ? ok : mk;
+    }
+    cand[r] = mw > 0.0f ? mk : kEmpty;
+    cw[r] = mw > 0.0f ? mw : 0.0f;
+    if (mw > 0.0f && bk == mk) {  // a label has one table entry: one lane holds the winner
+      bw = 0.0f;
+      bk = kEmpty;
+    }
+  }
+}
+
 // Lanes with `part` set (their vertex u has edges [b, e) with e - b <= CAP) get the sketch's
 // choice (cb, wb); MG = Misra-Gries 8 slots with certified-or-rescanned argmax (ties: see
 // argmax8), else Boyer-Moore.

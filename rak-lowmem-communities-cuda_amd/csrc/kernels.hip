@@ -30,6 +30,17 @@ namespace rlc {
 
 namespace {
 
+// With the rescan, the scan of the first iteration also writes the label it read at each edge
+// position (lbuf), and a rescan streams those back instead of gathering again: in that
+// iteration every label is distinct, no sketch choice is certified and nearly every vertex
+// rescans. Later iterations certify most choices, and the few rescans gather the labels again
+// (cheaper than writing 4 bytes per edge for all). Either way the rescan adds the same values
+// in the same order (synchronous mode: the snapshot labels).
+template <bool RESCAN>
+__device__ __forceinline__ bool store_labels(uint32_t iter) {
+  return RESCAN && iter == 0;
+}
+
 // ---- lane per vertex: one sequential sketch per lane ----------------------------------------
 // Items are 64-position chunks of the ranges of `rs`; one chunk per wave.
 template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
@@ -37,6 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
   const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
+  const bool store = store_labels<RESCAN>(iter);
   unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
   __shared__ unsigned int wcount[kBlock / 64];
   __shared__ unsigned long long bbase;
@@ -66,16 +78,19 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
         MG8 sk;
         sk.clear();
         auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
-        for_edges<UNIT, 1, decltype(acc_fn), RESCAN>(g, s.lab, u, b, e, acc_fn, s.lbuf);
+        if (store) for_edges<UNIT, 1, decltype(acc_fn), RESCAN>(g, s.lab, u, b, e, acc_fn, s.lbuf);
+        else for_edges<UNIT, 1>(g, s.lab, u, b, e, acc_fn);
         if (!RESCAN) {
           argmax8<!SYNC>(sk.c, sk.w, cb, wb);
         } else if (!certain_argmax<!SYNC>(sk.c, sk.w, sk.dsum, deg, cb, wb)) {  // rescan
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) sk.w[j] = 0.0f;
-          for_labels<UNIT, 1>(g, s.lbuf, b, e, [&](uint32_t c, float w) {
+          auto rs_fn = [&](uint32_t c, float w) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) sk.w[j] = sk.c[j] == c ? sk.w[j] + w : sk.w[j];
-          });
+          };
+          if (store) for_labels<UNIT, 1>(g, s.lbuf, b, e, rs_fn);
+          else for_edges<UNIT, 1>(g, s.lab, u, b, e, rs_fn);
           argmax8<!SYNC>(sk.c, sk.w, cb, wb);
         }
       } else {
@@ -140,6 +155,11 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
   __shared__ float sh_tot[kSlots];
   __shared__ uint32_t sh_flag;
   __shared__ float sh_d[GW], sh_dt;
+  // asynchronous groups inside one wave merge through an LDS table (group_merge_lds)
+  constexpr bool kLdsMerge = !SYNC && MG && GW == 1;
+  __shared__ uint32_t mt_k[kLdsMerge ? kBlock / 64 : 1][kLdsMerge ? kMergeWave : 1];
+  __shared__ float mt_w[kLdsMerge ? kBlock / 64 : 1][kLdsMerge ? kMergeWave : 1];
+  const bool store = store_labels<RESCAN>(iter);
   unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
   const uint64_t total = rs.acc[rs.count];
   for (uint64_t base = (uint64_t)blockIdx.x * VPB; base < total; base += (uint64_t)gridDim.x * VPB) {
@@ -170,12 +190,18 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
         MG8 sk;
         sk.clear();
         auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
-        for_edges<UNIT, P, decltype(acc_fn), RESCAN>(g, s.lab, u, b + gl, e, acc_fn, s.lbuf);
-        group_merge_tree<W>(sk);
+        if (store) for_edges<UNIT, P, decltype(acc_fn), RESCAN>(g, s.lab, u, b + gl, e, acc_fn, s.lbuf);
+        else for_edges<UNIT, P>(g, s.lab, u, b + gl, e, acc_fn);
         uint32_t cand[kSlots];
         float cw[kSlots];  // merged weights
         float dtot;        // decrement weight of the whole group (scans and absorbs)
-        if (GW == 1) {
+        if constexpr (kLdsMerge) {
+          group_merge_lds<W>(sk, mt_k[threadIdx.x >> 6], mt_w[threadIdx.x >> 6], cand, cw);
+          dtot = sk.dsum;
+#pragma unroll
+          for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
+        } else if (GW == 1) {
+          group_merge_tree<W>(sk);
           const int leader = lane & ~(W - 1);
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) {
@@ -186,6 +212,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
 #pragma unroll
           for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
         } else {
+          group_merge_tree<W>(sk);
           float wd = sk.dsum;
 #pragma unroll
           for (int k = 1; k < 64; k <<= 1) wd += __shfl_xor(wd, k);
@@ -231,10 +258,12 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
         } else {
 #pragma unroll
         for (int j = 0; j < kSlots; ++j) acc[j] = 0.0f;
-        for_labels<UNIT, P>(g, s.lbuf, b + gl, e, [&](uint32_t c, float w) {
+        auto rs_fn = [&](uint32_t c, float w) {
 #pragma unroll
           for (int j = 0; j < kSlots; ++j) acc[j] = cand[j] == c ? acc[j] + w : acc[j];
-        });
+        };
+        if (store) for_labels<UNIT, P>(g, s.lbuf, b + gl, e, rs_fn);
+        else for_edges<UNIT, P>(g, s.lab, u, b + gl, e, rs_fn);
         group_sum8<W>(acc);
         if (GW > 1) {
           if (lane == 0) {
