@@ -371,7 +371,12 @@ __device__ __forceinline__ void staged_choose(const DevGraph &g, const uint32_t 
 }
 
 // ---- synchronous mode: commit the change list ------------------------------------------------
-// Lane per change for the label write; the wave then marks each change's neighbours together.
+// Lane per change for the label write. A change of degree at most kApplyLaneDegree has its
+// neighbours marked by its own lane (every lane keeps its own loads in flight); the wave marks
+// the neighbours of each larger one together, 64 positions at a time. Marks are idempotent, so
+// the order does not matter.
+constexpr uint64_t kApplyLaneDegree = 64;
+
 __global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
   if (s.ctr->done) return;
   const unsigned long long nm = s.ctr->nmoves;
@@ -392,8 +397,11 @@ __global__ __launch_bounds__(kBlock) void k_apply(DevGraph g, DevState s) {
       e = g.off[(uint64_t)u + 1];
       sx += e - b;
     }
-    const int cnt = (int)((nm - base) < 64 ? (nm - base) : 64);
-    for (int j = 0; j < cnt; ++j) {
+    if (idx < nm && e - b <= kApplyLaneDegree) mark_neighbours(g, s.aff, u, b, e, 1);
+    unsigned long long big = __ballot(idx < nm && e - b > kApplyLaneDegree);
+    while (big) {
+      const int j = __ffsll((long long)big) - 1;
+      big &= big - 1;
       const uint32_t uj = __shfl(u, j);
       const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
       mark_neighbours(g, s.aff, uj, bj + lane, ej, 64);
