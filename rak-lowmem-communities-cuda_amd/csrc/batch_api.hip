@@ -95,8 +95,53 @@ BatchTune rlc::make_tune(const rlc_batch *b, const rlc_options &o, const rlc_bat
   return bt;
 }
 
+rlc_status rlc::PinnedStage::init() {
+  if (hipHostMalloc((void **)&buf, kChunkBytes * kChunks, hipHostMallocDefault) != hipSuccess) {
+    buf = nullptr;
+    return fail(RLC_ERR_OUT_OF_MEMORY, "pinned staging buffer");
+  }
+  for (int k = 0; k < kChunks; ++k)
+    if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(RLC_ERR_DEVICE, "pinned staging events");
+  return RLC_OK;
+}
+
+void rlc::PinnedStage::destroy() {
+  for (int k = 0; k < kChunks; ++k) {
+    if (ev[k]) {
+      if (pending[k]) (void)hipEventSynchronize(ev[k]);
+      (void)hipEventDestroy(ev[k]);
+      ev[k] = nullptr;
+    }
+    pending[k] = false;
+  }
+  if (buf) (void)hipHostFree(buf);
+  buf = nullptr;
+}
+
+rlc_status rlc::PinnedStage::copy(void *dst, const void *src, uint64_t bytes, hipStream_t st) {
+  char *d = (char *)dst;
+  const char *s = (const char *)src;
+  while (bytes) {
+    const uint64_t len = bytes < kChunkBytes ? bytes : kChunkBytes;
+    const int k = next;
+    next = (next + 1) % kChunks;
+    if (pending[k] && hipEventSynchronize(ev[k]) != hipSuccess) return fail(RLC_ERR_DEVICE, "staging: copy failed");
+    char *p = buf + (uint64_t)k * kChunkBytes;
+    parallel_memcpy(p, s, len, kThreads);
+    if (hipMemcpyAsync(d, p, len, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventRecord(ev[k], st) != hipSuccess)
+      return fail(RLC_ERR_DEVICE, "staging: copy failed");
+    pending[k] = true;
+    d += len;
+    s += len;
+    bytes -= len;
+  }
+  return RLC_OK;
+}
+
 rlc_status rlc::batch_place(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg,
-                            const float *wei, hipStream_t st, uint32_t *id) {
+                            const float *wei, hipStream_t st, uint32_t *id, PinnedStage *stage) {
   if (wei && !b->weighted) {
     for (uint64_t i = 0; i < m; ++i)
       if (wei[i] != 1.0f) return fail(RLC_ERR_INVALID_ARGUMENT, "batch: the arena stores no weights (weighted = 0)");
@@ -115,9 +160,12 @@ rlc_status rlc::batch_place(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t
     return s;
   };
   b->slot_stage[g] = BatchSlot{(uint32_t)vb, (uint32_t)n, 0u, (uint32_t)kSlotIdle, 0ull, 0ull};
-  if (hipMemcpyAsync(b->d_off + vb, off, (n + 1) * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
-      (m && hipMemcpyAsync(b->d_edg + eb, edg, m * 4, hipMemcpyHostToDevice, st) != hipSuccess) ||
-      (m && b->weighted && wei && hipMemcpyAsync(b->d_wei + eb, wei, m * 4, hipMemcpyHostToDevice, st) != hipSuccess) ||
+  auto up = [&](void *dst, const void *src, uint64_t bytes) {
+    if (stage) return stage->copy(dst, src, bytes, st) == RLC_OK;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st) == hipSuccess;
+  };
+  if (!up(b->d_off + vb, off, (n + 1) * 8) || (m && !up(b->d_edg + eb, edg, m * 4)) ||
+      (m && b->weighted && wei && !up(b->d_wei + eb, wei, m * 4)) ||
       hipMemcpyAsync(b->d_slots + g, &b->slot_stage[g], sizeof(BatchSlot), hipMemcpyHostToDevice, st) != hipSuccess)
     return undo(fail(RLC_ERR_DEVICE, "batch: graph copy failed"));
   launch_batch_rebase(b->d_off + vb, n, eb, b->d_edg + eb, m, (uint32_t)vb, b->weighted && !wei ? b->d_wei + eb : nullptr,

@@ -118,11 +118,29 @@ rlc_status check_batch_options(const rlc_options *o);
 BatchTune make_tune(const rlc_batch *b, const rlc_options &o, const rlc_batch_tuning &t);
 // The change list of the synchronous mode (allocated on first use).
 rlc_status ensure_moves(rlc_batch *b, const rlc_options &o);
+// Pinned host staging for uploads from pageable memory: a ring of kChunks chunks of kChunkBytes.
+// A chunk is refilled (host copy, several threads for large pieces) once the device copy that
+// last read it has completed (its event), then copied to the device asynchronously on the
+// caller's stream, so the host copy of the next chunk overlaps the transfer of this one.
+struct PinnedStage {
+  static constexpr uint64_t kChunkBytes = 32ull << 20;
+  static constexpr int kChunks = 4;
+  static constexpr int kThreads = 8;
+  char *buf = nullptr;
+  hipEvent_t ev[kChunks] = {};
+  bool pending[kChunks] = {};
+  int next = 0;
+  rlc_status init();
+  void destroy();  // waits for the copies in flight
+  rlc_status copy(void *dst, const void *src, uint64_t bytes, hipStream_t st);
+};
+
 // Takes a graph id and arena ranges for a validated CSR and enqueues its copy into the arena
-// (local values rebased to arena values) and its slot entry (idle) on stream st. The caller
+// (local values rebased to arena values) and its slot entry (idle) on stream st; the arrays go
+// through `stage` when one is given, else straight from the caller's memory. The caller
 // synchronises st before the graph is used; on failure nothing is taken.
 rlc_status batch_place(rlc_batch *b, uint64_t n, uint64_t m, const uint64_t *off, const uint32_t *edg,
-                       const float *wei, hipStream_t st, uint32_t *id);
+                       const float *wei, hipStream_t st, uint32_t *id, PinnedStage *stage = nullptr);
 // Gives graph id's ranges and id back (host bookkeeping only).
 void batch_release(rlc_batch *b, uint32_t id);
 

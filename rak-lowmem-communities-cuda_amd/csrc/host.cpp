@@ -4,6 +4,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <cstring>
 #include <string>
 
 namespace rlc {
@@ -44,5 +45,21 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
 }
 
 uint64_t segment_of(uint64_t u, uint64_t S) { return S <= 1 ? 0 : (u / kSegmentWindow) % S; }
+
+void parallel_memcpy(void *dst, const void *src, uint64_t bytes, int threads) {
+  constexpr uint64_t kPiece = 1ull << 20;
+  const uint64_t pieces = (bytes + kPiece - 1) / kPiece;
+  if (threads <= 1 || pieces < 4) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  char *d = (char *)dst;
+  const char *s = (const char *)src;
+#pragma omp parallel for schedule(static) num_threads(threads)
+  for (int64_t k = 0; k < (int64_t)pieces; ++k) {
+    const uint64_t a = (uint64_t)k * kPiece, len = std::min(kPiece, bytes - a);
+    std::memcpy(d + a, s + a, len);
+  }
+}
 
 }  // namespace rlc

@@ -20,4 +20,7 @@ rlc_status validate_csr(uint64_t n, uint64_t m, const uint64_t *off, const uint3
 constexpr uint64_t kSegmentWindow = 8;
 uint64_t segment_of(uint64_t u, uint64_t S);
 
+// memcpy with up to `threads` OpenMP threads for large copies (pieces of at least 1 MiB).
+void parallel_memcpy(void *dst, const void *src, uint64_t bytes, int threads);
+
 }  // namespace rlc

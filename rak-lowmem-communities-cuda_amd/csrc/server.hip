@@ -67,6 +67,7 @@ struct rlc_server {
   hipEvent_t ev_copy = nullptr, ev_round = nullptr;
   hipGraph_t hg = nullptr;
   hipGraphExec_t hx = nullptr;
+  PinnedStage stage;  // uploads from the callers' pageable memory go through pinned chunks
   // requests
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -111,7 +112,7 @@ void admit_queued(rlc_server *S) {
     }
     uint32_t gid = 0;
     const double t = S->now();
-    const rlc_status st = batch_place(b, r->n, r->m, r->off, r->edg, r->wei, S->copy, &gid);
+    const rlc_status st = batch_place(b, r->n, r->m, r->off, r->edg, r->wei, S->copy, &gid, &S->stage);
     std::lock_guard<std::mutex> lk(S->mu);
     if (st == RLC_ERR_CAPACITY) break;  // wait for space: arrival order is kept
     S->queue.pop_front();
@@ -253,6 +254,7 @@ void free_server(rlc_server *S) {
   if (S->ctx) (void)hipSetDevice(S->ctx->device);
   if (S->compute) (void)hipStreamSynchronize(S->compute);
   if (S->copy) (void)hipStreamSynchronize(S->copy);
+  S->stage.destroy();
   if (S->hx) (void)hipGraphExecDestroy(S->hx);
   if (S->hg) (void)hipGraphDestroy(S->hg);
   if (S->ev_copy) (void)hipEventDestroy(S->ev_copy);
@@ -309,6 +311,7 @@ rlc_status rlc_server_create(rlc_context *ctx, const rlc_server_config *c, const
       hipEventCreateWithFlags(&S->ev_copy, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&S->ev_round, hipEventDisableTiming) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: stream or event creation failed"));
+  if ((st = S->stage.init()) != RLC_OK) return bail(st);
   if ((st = ensure_moves(b, *o)) != RLC_OK) return bail(st);
   // the round: steps_per_round steps of every running graph, captured once
   const BatchTune bt = make_tune(b, *o, S->tune);
