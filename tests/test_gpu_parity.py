@@ -266,3 +266,59 @@ def test_exact_vote_async_quality(ctx, case):
     print(f"exact async {case}: Q {q:.4f} ({r.iterations} iterations), GVE-LPA {q_gve:.4f}", flush=True)
     assert 1 <= r.iterations <= 20
     assert q >= 0.98 * q_gve - 1e-3
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_async_group_merge_majority(ctx, weighted):
+    # Centers of degree 40 .. 3500 (4, 16 and 64 lanes per vertex) whose leaves carry labels 0
+    # (70% of the edges) and 1 (30%). Every lane sketch holds both labels exactly, so the group
+    # merge (an LDS table in the asynchronous mode) must give 0 unweighted; with weight 3 on the
+    # label-1 edges label 1 outweighs (0.9 d against 0.7 d). Centers start at label n-1 and, in the
+    # pick-less first iteration, may only move to a smaller label; leaves cannot move (their only
+    # neighbour's label is larger than theirs until the center moves).
+    degs = [40, 100, 300, 700, 2000, 3500]
+    K = len(degs)
+    pairs, w, init = [], [], []
+    leaf = K
+    for j, d in enumerate(degs):
+        for i in range(d):
+            lab = 1 if i % 10 < 3 else 0
+            pairs.append((j, leaf))
+            w.append(3.0 if (weighted and lab == 1) else 1.0)
+            init.append((leaf, lab))
+            leaf += 1
+    n = leaf
+    g = from_pairs(n, pairs, weights=w if weighted else None)
+    lab0 = np.full(n, n - 1, np.uint32)
+    for v, lab in init:
+        lab0[v] = lab
+    want = 1 if weighted else 0
+    for mode in (rlc.MODE_ASYNC, rlc.MODE_SYNC):
+        with ctx.upload(g) as dg:
+            r = dg.run(rlc.options(mode=mode, max_iterations=1), init=lab0)
+        assert r.iterations == 1
+        assert r.membership[:K].tolist() == [want] * K, (mode, r.membership[:K])
+
+
+def test_async_group_merge_many_labels(ctx):
+    # More than 8 labels per lane (sketches decrement) with one label on 40% of each center's edges
+    # and 12 others sharing the rest: the merged candidates must still yield the heavy label.
+    degs = [64, 256, 1024, 3000]
+    K = len(degs)
+    pairs, init = [], []
+    leaf = K
+    for j, d in enumerate(degs):
+        for i in range(d):
+            lab = 0 if i % 5 < 2 else 1 + (i * 7) % 12
+            pairs.append((j, leaf))
+            init.append((leaf, lab))
+            leaf += 1
+    n = leaf
+    g = from_pairs(n, pairs)
+    lab0 = np.full(n, n - 1, np.uint32)
+    for v, lab in init:
+        lab0[v] = lab
+    for mode in (rlc.MODE_ASYNC, rlc.MODE_SYNC):
+        with ctx.upload(g) as dg:
+            r = dg.run(rlc.options(mode=mode, max_iterations=1), init=lab0)
+        assert r.membership[:K].tolist() == [0] * K, (mode, r.membership[:K])
