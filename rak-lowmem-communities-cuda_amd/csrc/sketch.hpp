@@ -42,21 +42,27 @@ struct MG8 {
       has = has || m;
     }
     if (has) return;
-    // highest-index zero-weight slot, or -1; then either insert there or decrement all slots.
-    // Branch-free: every slot computes both outcomes and selects. The clamp max(x, 0) is
-    // fmaxf: weights are finite and never -0 here (x - y with x >= 0, y > 0 rounds to +0 when
-    // equal), so it equals the reference's std::max(x, 0).
-    int f = -1;
+    // Insert into the highest-index zero-weight slot, else decrement every slot (clamped at 0).
+    // Branch-free, with lane-mask booleans for the slot choice (scalar instructions) instead of a
+    // serial index chain: sel[i] = slot i is the highest zero slot. The decrement is applied as
+    // max(w - dec, 0) with dec = 0 when a slot is free: w - 0 = w and max(w, 0) = w exactly, as
+    // weights are finite and never -0 here (x - y with x >= 0, y > 0 rounds to +0 when equal);
+    // fmaxf equals the reference's std::max(x, 0) on them.
+    bool sel[kSlots];
+    bool taken = false;
 #pragma unroll
-    for (int i = 0; i < kSlots; ++i) f = (w[i] == 0.0f) ? i : f;
-    const bool full = f < 0;
-    dsum = full ? dsum + wt : dsum;
+    for (int i = kSlots - 1; i >= 0; --i) {
+      const bool z = w[i] == 0.0f;
+      sel[i] = z && !taken;
+      taken = taken || z;
+    }
+    const float dec = taken ? 0.0f : wt;
+    dsum += dec;
 #pragma unroll
     for (int i = 0; i < kSlots; ++i) {
-      const bool sel = i == f;
-      const float x = fmaxf(w[i] - wt, 0.0f);
-      c[i] = sel ? lab : c[i];
-      w[i] = sel ? wt : (full ? x : w[i]);
+      const float x = fmaxf(w[i] - dec, 0.0f);
+      c[i] = sel[i] ? lab : c[i];
+      w[i] = sel[i] ? wt : x;
     }
   }
 
