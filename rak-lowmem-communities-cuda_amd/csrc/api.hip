@@ -79,6 +79,7 @@ struct rlc_graph {
   double *d_modout = nullptr;
   bool labels_dense = true;
   hipStream_t stream = nullptr;
+  StreamFork fork;  // concurrent degree classes of a sub-sweep (asynchronous mode)
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t evm[2] = {nullptr, nullptr};
   hipGraph_t hgraph = nullptr;
@@ -121,6 +122,14 @@ void free_graph(rlc_graph *g) {
     if (e) (void)hipEventDestroy(e);
   for (auto &e : g->evm)
     if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < StreamFork::kAux; ++k) {
+    if (g->fork.aux[k]) {
+      (void)hipStreamSynchronize(g->fork.aux[k]);
+      (void)hipStreamDestroy(g->fork.aux[k]);
+    }
+    if (g->fork.join[k]) (void)hipEventDestroy(g->fork.join[k]);
+  }
+  if (g->fork.fork) (void)hipEventDestroy(g->fork.fork);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }
@@ -275,7 +284,7 @@ Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
   const bool one_sweep = o->sweeps == 1;
   for (size_t s = 0; s < (one_sweep ? 1 : S); ++s) {
     for (int c = 1; c < 5; ++c) {
-      MoveLaunch L{kLanes[c], empty_ranges()};
+      MoveLaunch L{kLanes[c], empty_ranges(), (int)s};
       for (size_t q = one_sweep ? 0 : s; q < (one_sweep ? S : s + 1); ++q)
         add_range(L.ranges, cuts[q][c], cuts[q][c + 1], kLanes[c] == 1);
       sc.launches.push_back(L);
@@ -428,6 +437,12 @@ static rlc_status new_graph(rlc_context *ctx, rlc_graph **out) {
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
   for (auto &e : g->evm)
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
+  for (int k = 0; k < StreamFork::kAux; ++k)
+    if (hipStreamCreateWithFlags(&g->fork.aux[k], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&g->fork.join[k], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(RLC_ERR_DEVICE, "stream or event creation failed"));
+  if (hipEventCreateWithFlags(&g->fork.fork, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
   *out = g;
   return RLC_OK;
 }
@@ -640,7 +655,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
       if (g->hgraph) { (void)hipGraphDestroy(g->hgraph); g->hgraph = nullptr; }
       g->key.valid = false;
       HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-      for (int l = 0; l < o->max_iterations; ++l) launch_iteration(dg, ds, cl, lp, g->stream, nullptr, nullptr);
+      for (int l = 0; l < o->max_iterations; ++l) launch_iteration(dg, ds, cl, lp, g->stream, nullptr, nullptr, &g->fork);
       hipGraph_t hg = nullptr;
       hipError_t ce = hipStreamEndCapture(g->stream, &hg);
       if (ce != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -664,7 +679,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
       if (o->max_iterations > 0) HIP_TRY(hipGraphLaunch(g->hexec, g->stream));
     } else {
       for (int l = 0; l < o->max_iterations; ++l) {
-        launch_iteration(dg, ds, cl, lp, g->stream, g->evm[0], g->evm[1]);
+        launch_iteration(dg, ds, cl, lp, g->stream, g->evm[0], g->evm[1], &g->fork);
         if ((st = check_launch()) != RLC_OK) return st;
         unsigned int done = 0;
         HIP_TRY(hipMemcpyAsync(&done, &g->d_ctr->done, sizeof(done), hipMemcpyDeviceToHost, g->stream));

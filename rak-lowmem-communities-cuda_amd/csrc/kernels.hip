@@ -20,7 +20,9 @@
 // list that the apply pass commits after the iteration.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "edges.hpp"
 #include "lpa.hpp"
@@ -926,7 +928,7 @@ void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, h
 
 template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
 static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
-                         hipStream_t st) {
+                         hipStream_t st, const StreamFork *fk) {
   if (!SYNC && MG && sc.chunked && sc.plan.nchunks > 0) {
     const unsigned gr = grid_for(sc.plan.nchunks, kBlock / 64, kGridCap);
     const unsigned gv = grid_for(sc.plan.nslots, kBlock / 64, kGridCap);
@@ -936,43 +938,66 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     k_chunk_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
     k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
   }
-  for (const MoveLaunch &L : sc.launches) {
+  auto launch = [&](const MoveLaunch &L, hipStream_t q) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
-    if (items == 0) continue;
     switch (L.lanes) {
       case 1:
-        k_move_lane<SYNC, MG, UNIT, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
-                                                                                             p.pickstep);
+        k_move_lane<SYNC, MG, UNIT, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
+                                                                                            p.pickstep);
         break;
       case 4:
-        k_move_group<SYNC, MG, UNIT, 4, RESCAN><<<grid_for(items, kBlock / 4, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
-                                                                                                p.pickstep);
+        k_move_group<SYNC, MG, UNIT, 4, RESCAN><<<grid_for(items, kBlock / 4, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
+                                                                                               p.pickstep);
         break;
       case 16:
-        k_move_group<SYNC, MG, UNIT, 16, RESCAN><<<grid_for(items, kBlock / 16, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
-                                                                                                  p.pickstep);
+        k_move_group<SYNC, MG, UNIT, 16, RESCAN><<<grid_for(items, kBlock / 16, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
+                                                                                                 p.pickstep);
         break;
       case 64:
-        k_move_group<SYNC, MG, UNIT, 64, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
-                                                                                                  p.pickstep);
+        k_move_group<SYNC, MG, UNIT, 64, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
+                                                                                                 p.pickstep);
         break;
       default:
-        k_move_group<SYNC, MG, UNIT, 256, RESCAN><<<grid_for(items, 1, kGridCap), kBlock, 0, st>>>(g, s, L.ranges,
-                                                                                         p.pickstep);
+        k_move_group<SYNC, MG, UNIT, 256, RESCAN><<<grid_for(items, 1, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
+                                                                                        p.pickstep);
         break;
     }
+  };
+  const std::vector<MoveLaunch> &Ls = sc.launches;
+  for (size_t i = 0; i < Ls.size();) {
+    size_t j = i + 1;
+    while (j < Ls.size() && Ls[i].group >= 0 && Ls[j].group == Ls[i].group) ++j;
+    std::vector<size_t> ne;  // the group's non-empty launches
+    for (size_t q = i; q < j; ++q)
+      if (Ls[q].ranges.acc[Ls[q].ranges.count] > 0) ne.push_back(q);
+    const int k = (int)std::min<size_t>(ne.size(), StreamFork::kAux + 1);
+    if (!SYNC && fk && k > 1) {
+      // the classes of a sub-sweep touch disjoint vertices: they run together, then join
+      (void)hipEventRecord(fk->fork, st);
+      for (int a = 1; a < k; ++a) {
+        (void)hipStreamWaitEvent(fk->aux[a - 1], fk->fork, 0);
+        launch(Ls[ne[a]], fk->aux[a - 1]);
+        (void)hipEventRecord(fk->join[a - 1], fk->aux[a - 1]);
+      }
+      launch(Ls[ne[0]], st);
+      for (int a = 1; a < k; ++a) (void)hipStreamWaitEvent(st, fk->join[a - 1], 0);
+      for (size_t a = (size_t)k; a < ne.size(); ++a) launch(Ls[ne[a]], st);
+    } else {
+      for (size_t q : ne) launch(Ls[q], st);
+    }
+    i = j;
   }
 }
 
 template <bool SYNC, bool MG>
 static void launch_moves_w(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
-                           hipStream_t st) {
+                           hipStream_t st, const StreamFork *fk) {
   if (g.wei) {
-    if (p.rescan || !MG) launch_moves<SYNC, MG, false, true>(g, s, sc, p, st);
-    else launch_moves<SYNC, MG, false, false>(g, s, sc, p, st);
+    if (p.rescan || !MG) launch_moves<SYNC, MG, false, true>(g, s, sc, p, st, fk);
+    else launch_moves<SYNC, MG, false, false>(g, s, sc, p, st, fk);
   } else {
-    if (p.rescan || !MG) launch_moves<SYNC, MG, true, true>(g, s, sc, p, st);
-    else launch_moves<SYNC, MG, true, false>(g, s, sc, p, st);
+    if (p.rescan || !MG) launch_moves<SYNC, MG, true, true>(g, s, sc, p, st, fk);
+    else launch_moves<SYNC, MG, true, false>(g, s, sc, p, st, fk);
   }
 }
 
@@ -989,7 +1014,7 @@ static void launch_ht(const DevGraph &g, const DevState &s, const Schedule &sc, 
 }
 
 void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
-                      hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1) {
+                      hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1, const StreamFork *fk) {
   if (ev_move0) (void)hipEventRecord(ev_move0, st);
   if (p.exact) {
     if (p.sync) {
@@ -1000,11 +1025,11 @@ void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, 
       else launch_ht<false, true>(g, s, cl, p, st);
     }
   } else if (p.sync) {
-    if (p.mg) launch_moves_w<true, true>(g, s, cl, p, st);
-    else launch_moves_w<true, false>(g, s, cl, p, st);
+    if (p.mg) launch_moves_w<true, true>(g, s, cl, p, st, fk);
+    else launch_moves_w<true, false>(g, s, cl, p, st, fk);
   } else {
-    if (p.mg) launch_moves_w<false, true>(g, s, cl, p, st);
-    else launch_moves_w<false, false>(g, s, cl, p, st);
+    if (p.mg) launch_moves_w<false, true>(g, s, cl, p, st, fk);
+    else launch_moves_w<false, false>(g, s, cl, p, st, fk);
   }
   if (ev_move1) (void)hipEventRecord(ev_move1, st);
   if (p.sync) k_apply<<<grid_for(g.n, kBlock, 4096), kBlock, 0, st>>>(g, s);
