@@ -79,7 +79,7 @@ struct rlc_graph {
   double *d_modout = nullptr;
   bool labels_dense = true;
   hipStream_t stream = nullptr;
-  StreamFork fork;  // concurrent degree classes of a sub-sweep (asynchronous mode)
+  StreamFork fork;  // concurrent degree classes of a sub-sweep
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t evm[2] = {nullptr, nullptr};
   hipGraph_t hgraph = nullptr;
@@ -247,7 +247,7 @@ Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
   }
   if (o->mode == RLC_MODE_SYNC) {
     const uint64_t D = o->merge_degree > 0 ? (uint64_t)o->merge_degree : ~0ull;
-    MoveLaunch grp{o->merge_parts, empty_ranges()}, lane{1, empty_ranges()};
+    MoveLaunch grp{o->merge_parts, empty_ranges(), 0}, lane{1, empty_ranges(), 0};  // run together
     for (size_t s = 0; s < S; ++s) {
       const uint64_t sb = g->h_seg[s], se = g->h_seg[s + 1];
       const uint64_t cut = D == ~0ull ? sb : first_below(g, sb, se, D);

@@ -971,8 +971,10 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     for (size_t q = i; q < j; ++q)
       if (Ls[q].ranges.acc[Ls[q].ranges.count] > 0) ne.push_back(q);
     const int k = (int)std::min<size_t>(ne.size(), StreamFork::kAux + 1);
-    if (!SYNC && fk && k > 1) {
-      // the classes of a sub-sweep touch disjoint vertices: they run together, then join
+    if (fk && k > 1) {
+      // the classes of a sub-sweep touch disjoint vertices: they run together, then join (the
+      // synchronous mode reads a snapshot and appends to the change list with atomics, so its
+      // result does not depend on the order either)
       (void)hipEventRecord(fk->fork, st);
       for (int a = 1; a < k; ++a) {
         (void)hipStreamWaitEvent(fk->aux[a - 1], fk->fork, 0);

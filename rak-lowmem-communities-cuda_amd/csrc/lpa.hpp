@@ -50,16 +50,16 @@ struct RangeSet {
 };
 
 // One move launch: `lanes` lanes per vertex (1, 4, 16, 64 or 256) over a range set.
-// Consecutive launches with the same `group` (a sub-sweep's degree classes in the asynchronous
-// mode) may run concurrently.
+// Consecutive launches with the same `group` (the degree classes of a sub-sweep) may run
+// concurrently.
 struct MoveLaunch {
   int lanes;
   RangeSet ranges;
   int group = -1;
 };
 
-// Streams that the degree classes of one sub-sweep run on together (asynchronous mode): the first
-// class on the caller's stream, the others on aux[], forked from and joined back to it by events.
+// Streams that the degree classes of one sub-sweep run on together: the first class on the
+// caller's stream, the others on aux[], forked from and joined back to it by events.
 struct StreamFork {
   static constexpr int kAux = 3;
   hipStream_t aux[kAux] = {};
@@ -114,7 +114,7 @@ void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *in
 void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, hipStream_t st);
 // One iteration: move launches for each non-empty class, the apply pass (synchronous mode) and
 // the finish step. ev_move0/ev_move1 (may be null) bracket the move launches.
-// fk (may be null): the sub-sweep classes of the asynchronous mode run concurrently on its streams.
+// fk (may be null): the degree classes of each sub-sweep run concurrently on its streams.
 void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, const LoopParams &p,
                       hipStream_t st, hipEvent_t ev_move0, hipEvent_t ev_move1, const StreamFork *fk = nullptr);
 // Work items of the modularity pass: (window of 64 consecutive positions, first edge of a piece
