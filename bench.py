@@ -410,6 +410,17 @@ def main():
             "algorithmic_bytes_per_step": total_bytes / args.steps, "move_ms_per_step": move_ms / args.steps,
         },
     }
+    # ceiling of the access pattern itself: one pass of the targets (and weights) of every edge
+    # position with the label gathers, no sketch arithmetic (rlc_probe_gather), against the
+    # first iteration (every vertex active: every position scanned once)
+    probe_ms = dg.probe_gather(5)
+    ceil = g.m / (probe_ms * 1e-3) if probe_ms > 0 else 0.0
+    it0 = st.edges[0] / (st.iter_ms[0] * 1e-3) if st.iter_ms[0] > 0 else 0.0
+    line["roofline"]["gather_probe"] = {
+        "ms_per_pass": probe_ms, "edges_per_s": ceil, "first_iteration_edges_per_s": it0,
+        "first_iteration_fraction": it0 / ceil if ceil > 0 else None,
+    }
+    log(f"gather probe: {probe_ms:.3f} ms per pass ({ceil:.3g} edges/s); first iteration {it0:.3g} edges/s")
     dg.close()
     ctx.close()
     if args.calibrate_copy:

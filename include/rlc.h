@@ -203,6 +203,13 @@ rlc_status rlc_graph_vertex_order(const rlc_graph *g, uint32_t *order);
 rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels, double *modularity,
                           uint64_t *communities);
 
+/* Measurement probe, no counterpart in the reference: the memory pattern of a move iteration
+ * without its sketch arithmetic. One pass streams every edge position of g in the device layout
+ * (target, and weight unless unit) and gathers the target's label, 8 gathers in flight per lane;
+ * *ms_per_pass is the mean of `passes` passes (HIP events, after one untimed pass). It reads the
+ * resident labels and changes nothing. */
+rlc_status rlc_probe_gather(rlc_context *ctx, rlc_graph *g, int32_t passes, float *ms_per_pass);
+
 /* ---- batches: many graphs resident in one CSR arena ---------------------------------------
  * No counterpart in the reference, which runs one graph per call. Graphs are added to a device
  * arena once; rlc_batch_run advances any subset of them together, one label-propagation

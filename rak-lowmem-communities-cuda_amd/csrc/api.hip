@@ -740,6 +740,31 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   return RLC_OK;
 }
 
+rlc_status rlc_probe_gather(rlc_context *ctx, rlc_graph *g, int32_t passes, float *ms_per_pass) {
+  clear_error();
+  if (!ctx || !g || !ms_per_pass) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_probe_gather: null argument");
+  if (passes < 1) return fail(RLC_ERR_INVALID_ARGUMENT, "rlc_probe_gather: passes must be >= 1");
+  BusyGuard guard(g->busy);
+  if (!guard.ok) return fail(RLC_ERR_BUSY, "rlc_probe_gather: graph is in use by another call");
+  HIP_TRY(hipSetDevice(ctx->device));
+  uint32_t *sink = nullptr;
+  rlc_status st = dev_alloc(&sink, kProbeThreads);
+  if (st != RLC_OK) return st;
+  const DevGraph dg = g->dev();
+  launch_probe_gather(dg, g->d_lab, sink, g->stream);  // untimed: first touch
+  hipError_t e = hipEventRecord(g->ev[0], g->stream);
+  for (int32_t r = 0; r < passes && e == hipSuccess; ++r) launch_probe_gather(dg, g->d_lab, sink, g->stream);
+  if (e == hipSuccess) e = hipEventRecord(g->ev[1], g->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(g->ev[1]);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, g->ev[0], g->ev[1]);
+  if (e == hipSuccess) e = hipGetLastError();
+  (void)hipFree(sink);
+  if (e != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("rlc_probe_gather: ") + hipGetErrorString(e));
+  *ms_per_pass = ms / (float)passes;
+  return RLC_OK;
+}
+
 rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels, double *modularity,
                           uint64_t *communities) {
   clear_error();

@@ -1038,6 +1038,42 @@ void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, 
   k_finish<<<1, 64, 0, st>>>(s.ctr, g.n, p.tolerance, p.max_iterations, p.pickstep, p.sync ? 1 : 0);
 }
 
+namespace {
+// Every thread takes positions t, t + T, .., 8 of them per round (T = threads of the launch):
+// targets and weights loaded, then the 8 label gathers issued before any is used.
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_probe_gather(DevGraph g, const uint32_t *lab, uint32_t *sink) {
+  const uint64_t T = (uint64_t)gridDim.x * kBlock;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  uint32_t acc = 0;
+  float fa = 0.0f;
+  for (uint64_t i = t; i < g.m; i += 8 * T) {
+    uint32_t v[8], c[8];
+    float w[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t j = i + (uint64_t)k * T;
+      const uint64_t jj = j < g.m ? j : i;  // past the end: repeat position i
+      v[k] = g.edg[jj];
+      w[k] = UNIT ? 1.0f : g.wei[jj];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = lab[v[k]];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      acc ^= c[k];
+      fa += w[k];
+    }
+  }
+  sink[t] = acc ^ __float_as_uint(fa);
+}
+}  // namespace
+
+void launch_probe_gather(const DevGraph &g, const uint32_t *lab, uint32_t *sink, hipStream_t st) {
+  if (g.wei) k_probe_gather<false><<<kProbeThreads / kBlock, kBlock, 0, st>>>(g, lab, sink);
+  else k_probe_gather<true><<<kProbeThreads / kBlock, kBlock, 0, st>>>(g, lab, sink);
+}
+
 void launch_modularity(const DevGraph &g, const uint32_t *lab, const ModItems &mi, double *ctot, double *K,
                        uint8_t *seen, double *out, hipStream_t st) {
   k_mod_zero<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(ctot, K, seen, out, g.n);

@@ -132,4 +132,9 @@ constexpr uint64_t kModPieceEdges = 4096;
 void launch_modularity(const DevGraph &g, const uint32_t *lab, const ModItems &mi, double *ctot, double *K,
                        uint8_t *seen, double *out, hipStream_t st);
 
+// Gather probe: every edge position's target (and weight) streamed, its label gathered; sink:
+// one value per thread of the launch (kProbeThreads entries).
+constexpr uint32_t kProbeThreads = 8192u * 256u;
+void launch_probe_gather(const DevGraph &g, const uint32_t *lab, uint32_t *sink, hipStream_t st);
+
 }  // namespace rlc

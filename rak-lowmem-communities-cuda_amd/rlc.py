@@ -128,6 +128,7 @@ SIGNATURES = [
     ("rlc_graph_labels_device", _P, [_P]),
     ("rlc_graph_vertex_order", C.c_int, [_P, _P]),
     ("rlc_modularity", C.c_int, [_P, _P, _P, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    ("rlc_probe_gather", C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_float)]),
     ("rlc_batch_tuning_default", None, [C.POINTER(BatchTuning)]),
     ("rlc_batch_create", C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_int32, C.POINTER(_P)]),
     ("rlc_batch_footprint", C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int32, C.c_int32]),
@@ -609,6 +610,13 @@ class DeviceGraph:
         lab = None if labels is None else np.ascontiguousarray(labels, dtype=np.uint32)
         _check(lib().rlc_modularity(self.ctx.h, self.h, _ptr(lab), C.byref(q), C.byref(k)))
         return q.value, k.value
+
+    def probe_gather(self, passes: int = 5) -> float:
+        """ms per pass of the gather probe (rlc_probe_gather): targets and weights streamed, labels
+        gathered, no sketch arithmetic."""
+        ms = C.c_float()
+        _check(lib().rlc_probe_gather(self.ctx.h, self.h, passes, C.byref(ms)))
+        return float(ms.value)
 
 
 def format_result(r: RakResult, modularity: float, communities: int, technique: str, threads: int = 0) -> str:

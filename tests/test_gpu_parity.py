@@ -322,3 +322,18 @@ def test_async_group_merge_many_labels(ctx):
         with ctx.upload(g) as dg:
             r = dg.run(rlc.options(mode=mode, max_iterations=1), init=lab0)
         assert r.membership[:K].tolist() == [0] * K, (mode, r.membership[:K])
+
+
+def test_probe_gather(ctx):
+    # the measurement probe runs on any uploaded graph (resident labels, weighted or not) and
+    # leaves the labels unchanged
+    for g in (cfg1(), random_weighted(3000, 20000, seed=4), rlc.Graph(np.zeros(4, np.uint64), np.zeros(0, np.uint32), None)):
+        with ctx.upload(g) as dg:
+            r = dg.run(rlc.options(mode=rlc.MODE_SYNC))
+            ms = dg.probe_gather(3)
+            assert ms >= 0.0
+            q1 = dg.modularity()
+            assert q1 == dg.modularity(r.membership)
+    with pytest.raises(rlc.RlcError):
+        with ctx.upload(cfg1()) as dg:
+            dg.probe_gather(0)
