@@ -437,14 +437,24 @@ static rlc_status new_graph(rlc_context *ctx, rlc_graph **out) {
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
   for (auto &e : g->evm)
     if (hipEventCreate(&e) != hipSuccess) return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
-  for (int k = 0; k < StreamFork::kAux; ++k)
-    if (hipStreamCreateWithFlags(&g->fork.aux[k], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&g->fork.join[k], hipEventDisableTiming) != hipSuccess)
-      return bail(fail(RLC_ERR_DEVICE, "stream or event creation failed"));
-  if (hipEventCreateWithFlags(&g->fork.fork, hipEventDisableTiming) != hipSuccess)
-    return bail(fail(RLC_ERR_DEVICE, "hipEventCreate failed"));
   *out = g;
   return RLC_OK;
+}
+
+// The fork streams of a graph, created on its first run (a graph that is only uploaded holds
+// none). If they cannot be created, the run stays on the graph's one stream.
+static const StreamFork *ensure_fork(rlc_graph *g) {
+  StreamFork &f = g->fork;
+  if (f.fork) return &f;
+  for (int k = 0; k < StreamFork::kAux; ++k)
+    if ((!f.aux[k] && hipStreamCreateWithFlags(&f.aux[k], hipStreamNonBlocking) != hipSuccess) ||
+        (!f.join[k] && hipEventCreateWithFlags(&f.join[k], hipEventDisableTiming) != hipSuccess))
+      return nullptr;
+  if (hipEventCreateWithFlags(&f.fork, hipEventDisableTiming) != hipSuccess) {
+    f.fork = nullptr;
+    return nullptr;
+  }
+  return &f;
 }
 
 static rlc_status upload_common(rlc_context *ctx, uint64_t n, uint64_t m, std::vector<uint64_t> &&off,
@@ -636,6 +646,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, !o->no_rescan, o->pickstep, o->tolerance, o->max_iterations};
   lp.exact = o->slots == RLC_SKETCH_EXACT;
   const bool graph_mode = o->timing == RLC_TIMING_GRAPH;
+  const StreamFork *fk = ensure_fork(g);
 
   if (graph_mode && o->max_iterations > 0) {
     ExecKey k;
@@ -655,7 +666,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
       if (g->hgraph) { (void)hipGraphDestroy(g->hgraph); g->hgraph = nullptr; }
       g->key.valid = false;
       HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-      for (int l = 0; l < o->max_iterations; ++l) launch_iteration(dg, ds, cl, lp, g->stream, nullptr, nullptr, &g->fork);
+      for (int l = 0; l < o->max_iterations; ++l) launch_iteration(dg, ds, cl, lp, g->stream, nullptr, nullptr, fk);
       hipGraph_t hg = nullptr;
       hipError_t ce = hipStreamEndCapture(g->stream, &hg);
       if (ce != hipSuccess) return fail(RLC_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -679,7 +690,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
       if (o->max_iterations > 0) HIP_TRY(hipGraphLaunch(g->hexec, g->stream));
     } else {
       for (int l = 0; l < o->max_iterations; ++l) {
-        launch_iteration(dg, ds, cl, lp, g->stream, g->evm[0], g->evm[1], &g->fork);
+        launch_iteration(dg, ds, cl, lp, g->stream, g->evm[0], g->evm[1], fk);
         if ((st = check_launch()) != RLC_OK) return st;
         unsigned int done = 0;
         HIP_TRY(hipMemcpyAsync(&done, &g->d_ctr->done, sizeof(done), hipMemcpyDeviceToHost, g->stream));
