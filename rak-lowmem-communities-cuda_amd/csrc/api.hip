@@ -230,14 +230,25 @@ Schedule build_schedule(rlc_graph *g, const rlc_options *o, rlc_status *st) {
     // exact vote: SYNC one lane per vertex (deterministic sums); ASYNC a wavefront per vertex of
     // degree >= 32 (the reference's block kernel threshold, inc/rakCuda.hxx:318) and a lane per
     // vertex below, segment after segment
+    // (asynchronous: vertices of degree >= the hub threshold, group_degree[3] or 4096, take one
+    // wavefront per 4096-edge chunk, k_htc_*, first)
     const uint64_t D = o->mode == RLC_MODE_SYNC ? ~0ull : 32;
     const bool one_sweep = o->sweeps == 1 || o->mode == RLC_MODE_SYNC;
+    uint64_t hub = ~0ull;
+    if (o->mode != RLC_MODE_SYNC) {
+      hub = (uint64_t)(o->group_degree[3] > 0 ? o->group_degree[3] : kDefaultGroupDegree[3]);
+      if (hub < D) hub = D;
+      if ((*st = ensure_chunk_plan(g, hub)) != RLC_OK) return sc;
+      sc.chunked = true;
+      sc.plan = g->plan;
+    }
     for (size_t s = 0; s < (one_sweep ? 1 : S); ++s) {
       MoveLaunch wave{64, empty_ranges()}, lane{1, empty_ranges()};
       for (size_t q = one_sweep ? 0 : s; q < (one_sweep ? S : s + 1); ++q) {
         const uint64_t sb = g->h_seg[q], se = g->h_seg[q + 1];
         const uint64_t cut = D == ~0ull ? sb : first_below(g, sb, se, D);
-        add_range(wave.ranges, sb, cut, false);
+        const uint64_t hb = hub == ~0ull ? sb : first_below(g, sb, se, hub);
+        add_range(wave.ranges, hb, cut, false);
         add_range(lane.ranges, cut, se, true);
       }
       sc.launches.push_back(wave);

@@ -554,6 +554,121 @@ __global__ __launch_bounds__(kBlock) void k_ht_wave(DevGraph g, DevState s, Rang
   flush_stats(s.ctr, sa, se, sc, sx);
 }
 
+// ---- exact vote, asynchronous mode: hubs cut into chunks ---------------------------------------
+// Vertices of the chunk plan (degree >= the plan's threshold) take one wavefront per chunk for
+// their global table instead of one wavefront per vertex, in three launches: clear (each chunk
+// its share of the table: 2 chunk_edges slots from 2 chunk_edges ci; H < 2 deg, so the chunks
+// cover it), insert (each chunk its edges, with the atomics of k_ht_wave), and choose (one
+// wavefront per vertex: the first slot of strictly largest weight, as k_ht_wave). The marks are
+// k_chunk_mark's. Insertion order across chunks is not fixed, so positions and hence the choice
+// among equal weights may vary from run to run, as in the asynchronous mode anyway.
+__global__ __launch_bounds__(kBlock) void k_htc_clear(DevGraph g, DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t c = wave_global; c < cp.nchunks; c += nwaves) {
+    const uint32_t slot = cp.cslot[c], ci = cp.cidx[c];
+    const uint32_t u = cp.vpos[slot];
+    if (s.aff[u] == 0) continue;  // wave-uniform
+    const uint64_t b = g.off[u], deg = g.off[(uint64_t)u + 1] - b;
+    const uint64_t H = next_pow2_ht(deg) - 1;
+    const uint64_t j0 = (uint64_t)ci * 2 * cp.chunk_edges, j1 = j0 + 2ull * cp.chunk_edges < H ? j0 + 2ull * cp.chunk_edges : H;
+    for (uint64_t j = j0 + (uint64_t)lane; j < j1; j += 64) {
+      s.htk[2 * b + j] = 0;
+      s.htw[2 * b + j] = 0.0f;
+    }
+  }
+}
+
+template <bool UNIT>
+__global__ __launch_bounds__(kBlock) void k_htc_insert(DevGraph g, DevState s, ChunkPlan cp) {
+  if (s.ctr->done) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t c = wave_global; c < cp.nchunks; c += nwaves) {
+    const uint32_t slot = cp.cslot[c], ci = cp.cidx[c];
+    const uint32_t u = cp.vpos[slot];
+    if (s.aff[u] == 0) continue;
+    const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
+    const uint64_t H = next_pow2_ht(ve - vb) - 1, T = next_pow2_ht(H) - 1;
+    uint32_t *hk = s.htk + 2 * vb;
+    float *hv = s.htw + 2 * vb;
+    const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
+    const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
+    for_edges<UNIT, 64>(g, s.lab, u, b + (uint64_t)lane, e, [&](uint32_t cl, float w) {
+      const uint32_t k = cl + 1;
+      uint64_t i = k, di = 1;
+      for (uint64_t t = 0; t < H; ++t) {
+        const uint64_t j = i % H;
+        const uint32_t h = __hip_atomic_load(&hk[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (h == k || h == 0) {
+          const uint32_t old = h == k ? k : atomicCAS(&hk[j], 0u, k);
+          if (old == 0u || old == k) {
+            atomicAdd(&hv[j], w);
+            break;
+          }
+        }
+        i += di;
+        di = di * 2 + (k % T);
+      }
+    });
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_htc_choose(DevGraph g, DevState s, ChunkPlan cp, int pickstep) {
+  if (s.ctr->done) return;
+  const uint32_t iter = s.ctr->iter;
+  const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
+  for (uint64_t slot = wave_global; slot < cp.nslots; slot += nwaves) {
+    const uint32_t u = cp.vpos[slot];
+    const bool act = s.aff[u] != 0;
+    if (lane == 0) cp.act[slot] = act ? 1u : 0u;
+    if (!act) continue;
+    const uint64_t b = g.off[u], deg = g.off[(uint64_t)u + 1] - b;
+    const uint64_t H = next_pow2_ht(deg) - 1;
+    const uint32_t *hk = s.htk + 2 * b;
+    const float *hv = s.htw + 2 * b;
+    float bw = -1.0f;
+    uint64_t bj = ~0ull;
+    for (uint64_t j = (uint64_t)lane; j < H; j += 64) {
+      const float x = hv[j];
+      if (x > bw) {
+        bw = x;
+        bj = j;
+      }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const float ow = __shfl_xor(bw, o);
+      const uint64_t oj = __shfl_xor(bj, o);
+      if (ow > bw || (ow == bw && oj < bj)) {
+        bw = ow;
+        bj = oj;
+      }
+    }
+    const uint32_t k0 = bj < H ? hk[bj] : 0u;
+    const uint32_t d = s.lab[u];
+    const uint32_t cb = k0 - 1;
+    const bool moved = k0 != 0 && cb != d && !(pickless && cb > d);
+    if (lane == 0) {
+      s.aff[u] = 0;
+      if (moved) s.lab[u] = cb;
+      cp.moved[slot] = moved ? 1u : 0u;
+      sa += 1;
+      se += deg;
+      sc += moved ? 1 : 0;
+      sx += moved ? deg : 0;
+    }
+  }
+  flush_stats(s.ctr, sa, se, sc, sx);
+}
+
 // ---- asynchronous mode: vertices cut into chunks, one wavefront per chunk -------------------
 // Five launches, with kernel boundaries as the only synchronisation (on this GPU a device-scope
 // fence writes back or invalidates the whole L2 of an XCD, far more costly than a launch):
@@ -1005,6 +1120,14 @@ static void launch_moves_w(const DevGraph &g, const DevState &s, const Schedule 
 
 template <bool SYNC, bool UNIT>
 static void launch_ht(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p, hipStream_t st) {
+  if (!SYNC && sc.chunked && sc.plan.nchunks > 0) {
+    const unsigned gr = grid_for(sc.plan.nchunks, kBlock / 64, kGridCap);
+    const unsigned gv = grid_for(sc.plan.nslots, kBlock / 64, kGridCap);
+    k_htc_clear<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_htc_insert<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_htc_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
+    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+  }
   for (const MoveLaunch &L : sc.launches) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
     if (items == 0) continue;

@@ -254,13 +254,17 @@ def test_exact_vote_sync_bit_exact(ctx, case):
     assert np.array_equal(rlc.iteration_stats(r.stats), ost)
 
 
-@pytest.mark.parametrize("case", ["cfg1", "sbm1m", "hubs"])
+@pytest.mark.parametrize("case", ["cfg1", "sbm1m", "hubs", "hubs9k", "cfg1-chunked"])
 def test_exact_vote_async_quality(ctx, case):
-    # asynchronous exact vote against the reference's exact CPU method (GVE-LPA, rakStaticOmp)
-    g = {"cfg1": cfg1, "sbm1m": lambda: rlc.gen_sbm(1 << 20, 1024, 16.0, 0.8, 3),
-         "hubs": lambda: with_hubs(20000, hubs=20, hub_degree=4000, base_degree=16, seed=8, weighted=False)}[case]()
+    # asynchronous exact vote against the reference's exact CPU method (GVE-LPA, rakStaticOmp);
+    # vertices of degree >= group_degree[3] (default 4096) take the chunked hub path (k_htc_*):
+    # hubs9k has 20 of them, cfg1-chunked sends every vertex of degree >= 32 there
+    g = {"cfg1": cfg1, "cfg1-chunked": cfg1, "sbm1m": lambda: rlc.gen_sbm(1 << 20, 1024, 16.0, 0.8, 3),
+         "hubs": lambda: with_hubs(20000, hubs=20, hub_degree=4000, base_degree=16, seed=8, weighted=False),
+         "hubs9k": lambda: with_hubs(40000, hubs=20, hub_degree=9000, base_degree=16, seed=8, weighted=False)}[case]()
+    kw = {"group_degree": (0, 0, 0, 32)} if case == "cfg1-chunked" else {}
     with ctx.upload(g) as dg:
-        r = dg.run(rlc.options(slots=0, pickstep=4))
+        r = dg.run(rlc.options(slots=0, pickstep=4, **kw))
         q, k = dg.modularity()
     q_gve = O.modularity(g, O.rak_omp(g, threads=8)[0])
     print(f"exact async {case}: Q {q:.4f} ({r.iterations} iterations), GVE-LPA {q_gve:.4f}", flush=True)
@@ -337,3 +341,31 @@ def test_probe_gather(ctx):
     with pytest.raises(rlc.RlcError):
         with ctx.upload(cfg1()) as dg:
             dg.probe_gather(0)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_exact_vote_chunked_hub_majority(ctx, weighted):
+    # hubs of degree 5000 and 12000 (chunked: one wavefront per 4096 edges, shared global table)
+    # whose leaves carry labels 0 (70% of the edges) and 1 (30%; weight 3 when weighted): the exact
+    # vote must give 0, or 1 when weighted (0.9 d against 0.7 d), in the pick-less first iteration
+    degs = [5000, 12000]
+    K = len(degs)
+    pairs, w, init = [], [], []
+    leaf = K
+    for j, d in enumerate(degs):
+        for i in range(d):
+            lab = 1 if i % 10 < 3 else 0
+            pairs.append((j, leaf))
+            w.append(3.0 if (weighted and lab == 1) else 1.0)
+            init.append((leaf, lab))
+            leaf += 1
+    n = leaf
+    g = from_pairs(n, pairs, weights=w if weighted else None)
+    lab0 = np.full(n, n - 1, np.uint32)
+    for v, lab in init:
+        lab0[v] = lab
+    want = 1 if weighted else 0
+    for mode in (rlc.MODE_ASYNC, rlc.MODE_SYNC):
+        with ctx.upload(g) as dg:
+            r = dg.run(rlc.options(mode=mode, slots=0, max_iterations=1), init=lab0)
+        assert r.membership[:K].tolist() == [want] * K, (mode, r.membership[:K])
