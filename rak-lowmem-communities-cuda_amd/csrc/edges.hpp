@@ -204,7 +204,8 @@ __device__ __forceinline__ void wave_lds_sync() {
 // lane's decrements of the true weight), so certain_argmax(cand, cw, D, ..) keeps its meaning,
 // and a rescan gives exact weights as after the tree. The tree costs log2(W) rounds of 8
 // sequential sketch updates by the whole wave; this costs 8 table inserts per lane and 8
-// group-wide maximum reductions. tk / tw: this wave's kMergeWave entries.
+// group-wide maximum reductions. tk / tw: this wave's kMergeWave entries, the groups' tables
+// interleaved.
 constexpr int kMergeTable = 16;
 constexpr int kMergeWave = 64 * kMergeTable;
 
@@ -212,14 +213,18 @@ template <int W>
 __device__ __forceinline__ void group_merge_lds(const MG8 &s, uint32_t *tk, float *tw, uint32_t (&cand)[kSlots],
                                                 float (&cw)[kSlots]) {
   constexpr uint32_t T = (uint32_t)W * kMergeTable;  // entries per group
+  constexpr uint32_t G = 64 / W;                     // groups per wave
   constexpr int LOG2T = __builtin_ctz(T);
   const int lane = lane_id();
   const uint32_t gl = (uint32_t)(lane & (W - 1));
-  const uint32_t g0 = (uint32_t)(lane & ~(W - 1)) * kMergeTable;  // group's first entry
+  const uint32_t grp = (uint32_t)lane / W;
+  // entry e of group grp at e * G + grp: the groups' tables interleave, so that the lanes of a wave
+  // clearing or reading entry gl + q W of their groups touch 64 consecutive words (no bank conflict)
+  auto at = [&](uint32_t e) { return e * G + grp; };
 #pragma unroll
   for (int q = 0; q < kMergeTable; ++q) {
-    tk[g0 + gl + q * W] = kEmpty;
-    tw[g0 + gl + q * W] = 0.0f;
+    tk[at(gl + q * W)] = kEmpty;
+    tw[at(gl + q * W)] = 0.0f;
   }
   wave_lds_sync();
 #pragma unroll
@@ -228,9 +233,9 @@ __device__ __forceinline__ void group_merge_lds(const MG8 &s, uint32_t *tk, floa
       const uint32_t key = s.c[j];
       uint32_t h = (key * 0x9E3779B1u) >> (32 - LOG2T);
       for (uint32_t probe = 0; probe < T; ++probe) {  // at most 8 W keys in T = 16 W entries
-        const uint32_t prev = atomicCAS(&tk[g0 + h], kEmpty, key);
+        const uint32_t prev = atomicCAS(&tk[at(h)], kEmpty, key);
         if (prev == kEmpty || prev == key) {
-          atomicAdd(&tw[g0 + h], s.w[j]);
+          atomicAdd(&tw[at(h)], s.w[j]);
           break;
         }
         h = (h + 1) & (T - 1);
@@ -242,8 +247,8 @@ __device__ __forceinline__ void group_merge_lds(const MG8 &s, uint32_t *tk, floa
   uint32_t bk = kEmpty;
 #pragma unroll
   for (int q = 0; q < kMergeTable; ++q) {
-    const float v = tw[g0 + gl + q * W];
-    const uint32_t k = tk[g0 + gl + q * W];
+    const float v = tw[at(gl + q * W)];
+    const uint32_t k = tk[at(gl + q * W)];
     const bool take = v > bw || (v == bw && v > 0.0f && k > bk);
     bw = take ? v : bw;
     bk = take ? k : bk;
