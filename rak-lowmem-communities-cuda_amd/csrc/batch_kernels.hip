@@ -95,8 +95,7 @@ __device__ __forceinline__ void choose_group(const DevGraph &g, const uint32_t *
       cw[j] = __shfl(sk.w[j], leader);
     }
     float dtot = sk.dsum;
-#pragma unroll
-    for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
+    butterfly<W>(dtot, [](float &x, float y) { x += y; });
     if (!certain_argmax<FIRST_SEEN>(cand, cw, dtot, e - b, cb, wb)) {
       float acc[kSlots];
 #pragma unroll

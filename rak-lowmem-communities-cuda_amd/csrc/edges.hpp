@@ -256,16 +256,13 @@ __device__ __forceinline__ void group_merge_lds(const MG8 &s, uint32_t *tk, floa
   wave_lds_sync();  // every read done before the table is cleared for the next vertex
 #pragma unroll
   for (int r = 0; r < kSlots; ++r) {
-    float mw = bw;
-    uint32_t mk = bk;
-#pragma unroll
-    for (int k = 1; k < W; k <<= 1) {
-      const float ow = __shfl_xor(mw, k);
-      const uint32_t ok = __shfl_xor(mk, k);
-      const bool take = ow > mw || (ow == mw && ok > mk);
-      mw = take ? ow : mw;
-      mk = take ? ok : mk;
-    }
+    // the group's heaviest offer: (weight, label) packed as one 64-bit value, weight in the high
+    // half (a non-negative float orders as its bit pattern), so that the butterfly maximum takes
+    // the heavier offer and, among equal weights, the larger label
+    uint64_t m = ((uint64_t)__float_as_uint(bw) << 32) | bk;
+    butterfly<W>(m, [](uint64_t &a, uint64_t b) { a = b > a ? b : a; });
+    const float mw = __uint_as_float((uint32_t)(m >> 32));
+    const uint32_t mk = (uint32_t)m;
     cand[r] = mw > 0.0f ? mk : kEmpty;
     cw[r] = mw > 0.0f ? mw : 0.0f;
     if (mw > 0.0f && bk == mk) {  // a label has one table entry: one lane holds the winner

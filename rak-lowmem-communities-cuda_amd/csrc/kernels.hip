@@ -200,8 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
         if constexpr (kLdsMerge) {
           group_merge_lds<W>(sk, mt_k[threadIdx.x >> 6], mt_w[threadIdx.x >> 6], cand, cw);
           dtot = sk.dsum;
-#pragma unroll
-          for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
+          butterfly<W>(dtot, [](float &x, float y) { x += y; });
         } else if (GW == 1) {
           group_merge_tree<W>(sk);
           const int leader = lane & ~(W - 1);
@@ -211,13 +210,11 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
             cw[j] = __shfl(sk.w[j], leader);
           }
           dtot = sk.dsum;
-#pragma unroll
-          for (int k = 1; k < W; k <<= 1) dtot += __shfl_xor(dtot, k);
+          butterfly<W>(dtot, [](float &x, float y) { x += y; });
         } else {
           group_merge_tree<W>(sk);
           float wd = sk.dsum;
-#pragma unroll
-          for (int k = 1; k < 64; k <<= 1) wd += __shfl_xor(wd, k);
+          butterfly<64>(wd, [](float &x, float y) { x += y; });
           if (lane == 0) {
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) { sh_c[gw][j] = sk.c[j]; sh_w[gw][j] = sk.w[j]; }
@@ -699,8 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_scan(DevGraph g, DevState s, C
     for_edges<UNIT, 64, decltype(acc_fn), RESCAN>(g, s.lab, u, b + lane, e, acc_fn, s.lbuf);
     group_merge_tree<64>(sk);
     float d = sk.dsum;
-#pragma unroll
-    for (int k = 1; k < 64; k <<= 1) d += __shfl_xor(d, k);
+    butterfly<64>(d, [](float &x, float y) { x += y; });
     if (lane == 0) {
 #pragma unroll
       for (int j = 0; j < kSlots; ++j) { cp.pc[c * kSlots + j] = sk.c[j]; cp.pw[c * kSlots + j] = sk.w[j]; }
@@ -734,8 +730,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_merge(DevGraph g, DevState s, 
       if (lane == 0) acc.absorb(part.c, part.w);
       d += part.dsum;
     }
-#pragma unroll
-    for (int k = 1; k < 64; k <<= 1) d += __shfl_xor(d, k);
+    butterfly<64>(d, [](float &x, float y) { x += y; });
     d += __shfl(acc.dsum, 0);
     if (lane == 0) {
       const uint32_t u = cp.vpos[slot];

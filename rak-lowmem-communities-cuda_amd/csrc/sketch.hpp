@@ -161,10 +161,38 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Value of lane (lane ^ M), M a power of two below 64, without an LDS round trip where the
+// hardware has a direct path: DPP quad permutations for M = 1, 2 (a VALU operand modifier),
+// ds_swizzle in bit mode for M = 4 .. 16 (lane ^ M inside 32-lane halves), ds_bpermute for 32.
+template <int M>
+__device__ __forceinline__ uint32_t xor_lanes(uint32_t x) {
+  static_assert(M >= 1 && M <= 32 && (M & (M - 1)) == 0, "mask must be a power of two below 64");
+  if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);
+  else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);
+  else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (M << 10) | 0x1F);
+  else return (uint32_t)__shfl_xor((int)x, 32);
+}
+template <int M>
+__device__ __forceinline__ float xor_lanes(float x) { return __uint_as_float(xor_lanes<M>(__float_as_uint(x))); }
+template <int M>
+__device__ __forceinline__ uint64_t xor_lanes(uint64_t x) {
+  return ((uint64_t)xor_lanes<M>((uint32_t)(x >> 32)) << 32) | xor_lanes<M>((uint32_t)x);
+}
+
+// Butterfly over lane masks 1, 2, .., W/2 (W a power of two <= 64): f(x, value of lane ^ k)
+// for each k in increasing order, with compile-time masks.
+template <int W, int K = 1, class T, class F>
+__device__ __forceinline__ void butterfly(T &x, F f) {
+  if constexpr (K < W) {
+    f(x, xor_lanes<K>(x));
+    butterfly<W, 2 * K>(x, f);
+  }
+}
+
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long x) {
-#pragma unroll
-  for (int k = 32; k >= 1; k >>= 1) x += __shfl_xor(x, k);
-  return x;
+  uint64_t y = x;
+  butterfly<64>(y, [](uint64_t &a, uint64_t b) { a += b; });
+  return y;
 }
 
 // Number of set bits of `mask` in lanes below this one.
@@ -209,9 +237,7 @@ __device__ __forceinline__ void group_bm_tree(uint32_t &mc, float &mw) {
 template <int W>
 __device__ __forceinline__ void group_sum8(float (&a)[kSlots]) {
 #pragma unroll
-  for (int k = 1; k < W; k <<= 1)
-#pragma unroll
-    for (int j = 0; j < kSlots; ++j) a[j] = a[j] + __shfl_xor(a[j], k);
+  for (int j = 0; j < kSlots; ++j) butterfly<W>(a[j], [](float &x, float y) { x = x + y; });
 }
 
 }  // namespace rlc
