@@ -203,31 +203,30 @@ __device__ __forceinline__ uint32_t lanes_below(unsigned long long mask) {
 // Merge tree over groups of W lanes (W a power of two <= 64, groups aligned in the wave): at
 // level k (1, 2, 4, .., W/2) lane p of a group with p % 2k == 0 absorbs lane p+k. The result is
 // in the group's first lane; other lanes hold partial sketches.
-template <int W>
+template <int W, int K = 1>
 __device__ __forceinline__ void group_merge_tree(MG8 &s) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int k = 1; k < W; k <<= 1) {
+  if constexpr (K < W) {
+    // lane p with p % 2K == 0 reads lane p + K = p ^ K (bit K of p is clear): DPP / ds_swizzle
     uint32_t oc[kSlots];
     float ow[kSlots];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) {
-      oc[j] = __shfl_down(s.c[j], k);
-      ow[j] = __shfl_down(s.w[j], k);
+      oc[j] = xor_lanes<K>(s.c[j]);
+      ow[j] = xor_lanes<K>(s.w[j]);
     }
-    if ((lane & (2 * k - 1)) == 0) s.absorb(oc, ow);
+    if ((lane_id() & (2 * K - 1)) == 0) s.absorb(oc, ow);
+    group_merge_tree<W, 2 * K>(s);
   }
 }
 
 // Boyer-Moore max tree over groups of W lanes: lane p takes lane p+k's candidate if strictly heavier.
-template <int W>
+template <int W, int K = 1>
 __device__ __forceinline__ void group_bm_tree(uint32_t &mc, float &mw) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int k = 1; k < W; k <<= 1) {
-    const uint32_t oc = __shfl_down(mc, k);
-    const float ow = __shfl_down(mw, k);
-    if ((lane & (2 * k - 1)) == 0 && ow > mw) { mc = oc; mw = ow; }
+  if constexpr (K < W) {
+    const uint32_t oc = xor_lanes<K>(mc);  // lane p + K for the lanes that take (p % 2K == 0)
+    const float ow = xor_lanes<K>(mw);
+    if ((lane_id() & (2 * K - 1)) == 0 && ow > mw) { mc = oc; mw = ow; }
+    group_bm_tree<W, 2 * K>(mc, mw);
   }
 }
 
