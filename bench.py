@@ -180,7 +180,8 @@ def bench_batch(args):
             "vertices": nsum, "entries": m_total, "tuning": tuned.as_dict(),
             "one_by_one_graphs_per_s": one, "one_by_one_tuning": tuned_1.as_dict(),
             "rlc_run_graphs_per_s": run_rate, "rlc_run_sample": len(rsample),
-            "speedup_vs_one_by_one": value / max(one, run_rate),
+            "speedup_vs_one_launch_per_graph": value / one if one > 0 else None,
+            "speedup_vs_rlc_run": value / run_rate if run_rate > 0 else None,
             "identical_labels": f"{same}/{args.graphs}", "batch_steps": st.steps,
             "traversed_edges_per_run": st.traversed_edges,
             "rlc_run_checked_graphs": sorted(sample),
