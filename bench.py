@@ -310,7 +310,9 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
-    ap.add_argument("--workload", choices=["rmat", "grid", "batch", "serve"], default="rmat")
+    ap.add_argument("--workload", choices=["rmat", "grid", "sbm", "batch", "serve"], default="rmat",
+                    help="rmat: R-MAT 24 async (default); grid: 10^8-vertex road grid; sbm: the 65,536-vertex SBM "
+                         "in the deterministic synchronous mode; batch: 4096-graph arena; serve: continuous batching")
     ap.add_argument("--unit", action="store_true", help="rmat workload: unit weights (the scale-27 configuration)")
     ap.add_argument("--grid-side", type=int, default=10000, help="grid workload: lattice side (10000: 1e8 vertices)")
     ap.add_argument("--requests", type=int, default=1000, help="serve workload: requests")
@@ -344,7 +346,12 @@ def main():
     import rlc
 
     t0 = time.time()
-    if args.workload == "grid":
+    sync = args.workload == "sbm"
+    if args.workload == "sbm":
+        g = rlc.gen_sbm(65536, 256, 16.0, 0.8, 1)
+        name = ("SBM 65,536 vertices in 256 equal blocks, expected degree 16, 80% intra-block, unit weights, "
+                "deterministic synchronous mode")
+    elif args.workload == "grid":
         side = args.grid_side
         g = rlc.gen_grid(side, side, 0.10, 0.02, seed=1)
         name = (f"road-like grid {side}x{side} (4-neighbour lattice, 10% of lattice edges dropped, shortcuts "
@@ -356,7 +363,7 @@ def main():
                 f"{'unit weights' if args.unit else 'w~U[0.5,1.5)'}")
     log(f"graph: {name}: n={g.n} m={g.m} average degree {g.m / g.n:.2f} max_degree={int(g.degrees().max())} "
         f"({time.time() - t0:.1f}s to generate)")
-    opts = rlc.options(mode=rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
+    opts = rlc.options(mode=rlc.MODE_SYNC if sync else rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
                        group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps)
     ctx = rlc.Context(0)
     t0 = time.time()
@@ -401,7 +408,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{name}, {METHOD[args.slots]} async, tol 0.05, max 20 iterations",
+            "workload": f"{name}, {METHOD[args.slots]} {'sync' if sync else 'async'}, tol 0.05, max 20 iterations",
             "n": g.n, "m": g.m, "iterations": iters, "modularity": q, "communities": ncom,
         },
         "roofline": {
@@ -430,7 +437,25 @@ def main():
     if traffic is not None:
         line["roofline"]["traffic"] = traffic["bytes_per_iteration"]
         line["roofline"]["traffic_source"] = traffic["source"]
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and sync:
+        import oracle
+
+        cores = len(os.sched_getaffinity(0))
+        cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+        # the oracle's deterministic synchronous mode on the same graph, repeated for about 10 s
+        t_cpu, k, trav = 0.0, 0, 0
+        while t_cpu < 10.0:
+            t = time.perf_counter()
+            _, it, ost = oracle.lpa_sync(g, slots=args.slots, threads=cores)
+            t_cpu += time.perf_counter() - t
+            trav += int(np.asarray(ost)[:, 1].sum())
+            k += 1
+        line["cpu_baseline"] = {
+            "value": trav / t_cpu, "unit": "edges/s", "cores": cores, "kind": "port",
+            "sample": f"{k} runs of the oracle's deterministic synchronous mode (MG8) on the same graph, "
+                      f"{it} iterations each, {t_cpu:.1f} s",
+        }
+    elif not args.no_cpu_baseline:
         import oracle
 
         cores = len(os.sched_getaffinity(0))
