@@ -34,10 +34,11 @@ namespace {
 
 // With the rescan, the scan of the first iteration also writes the label it read at each edge
 // position (lbuf), and a rescan streams those back instead of gathering again: in that
-// iteration every label is distinct, no sketch choice is certified and nearly every vertex
-// rescans. Later iterations certify most choices, and the few rescans gather the labels again
-// (cheaper than writing 4 bytes per edge for all). Either way the rescan adds the same values
-// in the same order (synchronous mode: the snapshot labels).
+// iteration every label is distinct and few choices are certified (on R-MAT 24, vertices that
+// rescan hold 65% of the traversed edges). Later iterations certify most choices (there, ~0%),
+// and the few rescans gather the labels again, cheaper than writing 4 bytes per edge for all.
+// Either way the rescan adds the same values in the same order (synchronous mode: the snapshot
+// labels).
 template <bool RESCAN>
 __device__ __forceinline__ bool store_labels(uint32_t iter) {
   return RESCAN && iter == 0;
