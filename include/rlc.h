@@ -4,6 +4,10 @@
  *
  * Plain C types only: pointers, sizes, PODs. No exceptions cross this boundary; every
  * entry point returns an rlc_status and leaves a thread-local message for rlc_last_error().
+ * Threads: different graphs, batches and servers may be driven from different host threads at once
+ * (each owns its streams; graphs are captured thread-locally and no call uses the null stream). A
+ * graph or batch takes one call at a time (a concurrent second call on it returns RLC_ERR_BUSY); a
+ * server's entry points may be called from any thread.
  *
  * Reference interfaces replaced (puzzlef/rak-lowmem-communities-cuda):
  *   - rlc_options           <- RakOptions                              inc/rak.hxx:19-40

@@ -590,8 +590,8 @@ rlc_status rlc_graph_csr_get(const rlc_graph *g, uint64_t *offsets, uint32_t *ed
   const uint64_t n = g->n, m = g->m;
   std::vector<uint32_t> e(m);
   std::vector<float> w(weights && !g->unit ? m : 0);
-  if (m) HIP_TRY(hipMemcpy(e.data(), g->d_edg, m * 4, hipMemcpyDeviceToHost));
-  if (!w.empty()) HIP_TRY(hipMemcpy(w.data(), g->d_wei, m * 4, hipMemcpyDeviceToHost));
+  if (m) HIP_TRY(copy_to_host(e.data(), g->d_edg, m * 4, g->stream));
+  if (!w.empty()) HIP_TRY(copy_to_host(w.data(), g->d_wei, m * 4, g->stream));
   // rows back in caller order, targets back to caller ids (row order inside each row is kept)
   std::vector<uint64_t> pos(n);
   for (uint64_t p = 0; p < n; ++p) pos[g->h_perm[p]] = p;
@@ -726,10 +726,10 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   g->labels_dense = dense;
 
   DevCounters hc;
-  HIP_TRY(hipMemcpy(&hc, g->d_ctr, sizeof(hc), hipMemcpyDeviceToHost));
+  HIP_TRY(copy_to_host(&hc, g->d_ctr, sizeof(hc), g->stream));
   if (out_labels && n) {
     std::vector<uint32_t> lp(n);
-    HIP_TRY(hipMemcpy(lp.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(lp.data(), g->d_lab, n * sizeof(uint32_t), g->stream));
     for (uint64_t p = 0; p < n; ++p) out_labels[g->h_perm[p]] = lp[p];  // back to caller order
   }
   if (stats) {
@@ -829,7 +829,7 @@ rlc_status rlc_modularity(rlc_context *ctx, rlc_graph *g, const uint32_t *labels
     if (labels) {
       for (uint64_t p = 0; p < n; ++p) src[p] = labels[g->h_perm[p]];
     } else if (n) {
-      HIP_TRY(hipMemcpy(src.data(), g->d_lab, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIP_TRY(copy_to_host(src.data(), g->d_lab, n * sizeof(uint32_t), g->stream));
     }
     bool ok = true;
     for (uint64_t u = 0; u < n; ++u) ok &= src[u] < n;

@@ -403,7 +403,7 @@ rlc_status rlc_batch_run(rlc_batch *b, const uint32_t *ids, uint32_t count, cons
   if ((st = check_launch()) != RLC_OK) return st;
   if (stats) {
     DevCounters hc;
-    HIP_TRY(hipMemcpy(&hc, b->d_ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(&hc, b->d_ctr, sizeof(hc), b->stream));
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
     stats->graphs = (uint32_t)sel.size();
@@ -414,7 +414,7 @@ rlc_status rlc_batch_run(rlc_batch *b, const uint32_t *ids, uint32_t count, cons
     for (uint32_t id : sel) e += b->slots[id].m;
     stats->edges = e;
     std::vector<unsigned long long> bs(4ull * b->nbstats);
-    HIP_TRY(hipMemcpy(bs.data(), b->d_bstats, bs.size() * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(copy_to_host(bs.data(), b->d_bstats, bs.size() * 8, b->stream));
     uint64_t E = 0, C = 0;
     for (uint32_t i = 0; i < b->nbstats; ++i) {
       E += bs[4ull * i + 1];

@@ -33,6 +33,13 @@ rlc_status dev_alloc(T **p, uint64_t count) {
   return RLC_OK;
 }
 
+// Device-to-host copy ordered on the object's own stream. A blocking hipMemcpy would go through the
+// null stream, which HIP refuses while another host thread captures a graph on any stream.
+inline hipError_t copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 // Validates every field of rlc_options (shared by rlc_run and rlc_batch_run).
 rlc_status check_options(const rlc_options *o);
 

@@ -64,6 +64,7 @@ struct rlc_server {
   int steps = 2;
   Clock::time_point t0;
   hipStream_t compute = nullptr, copy = nullptr;
+  hipStream_t query = nullptr;  // stats reads from caller threads, ordered behind nothing of the worker
   hipEvent_t ev_copy = nullptr, ev_round = nullptr;
   hipGraph_t hg = nullptr;
   hipGraphExec_t hx = nullptr;
@@ -261,6 +262,7 @@ void free_server(rlc_server *S) {
   if (S->ev_round) (void)hipEventDestroy(S->ev_round);
   if (S->compute) (void)hipStreamDestroy(S->compute);
   if (S->copy) (void)hipStreamDestroy(S->copy);
+  if (S->query) (void)hipStreamDestroy(S->query);
   if (S->b) free_batch(S->b);
   delete S;
 }
@@ -308,6 +310,7 @@ rlc_status rlc_server_create(rlc_context *ctx, const rlc_server_config *c, const
   };
   if (hipStreamCreateWithFlags(&S->compute, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&S->copy, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->query, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&S->ev_copy, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&S->ev_round, hipEventDisableTiming) != hipSuccess)
     return bail(fail(RLC_ERR_DEVICE, "rlc_server_create: stream or event creation failed"));
@@ -413,11 +416,11 @@ rlc_status rlc_server_stats_get(rlc_server *S, rlc_server_stats *st) {
     st->steps = S->steps_run;
     st->resident_max = S->resident_max;
   }
-  // traversed edges: the move kernels' per-workgroup counters (read with a blocking copy on the
-  // null stream of this thread, which does not wait for the worker's streams)
+  // traversed edges: the move kernels' per-workgroup counters (read on the server's query stream,
+  // which waits for none of the worker's streams)
   std::vector<unsigned long long> bs(4ull * S->b->nbstats);
   HIP_TRY(hipSetDevice(S->ctx->device));
-  HIP_TRY(hipMemcpy(bs.data(), S->b->d_bstats, bs.size() * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(copy_to_host(bs.data(), S->b->d_bstats, bs.size() * 8, S->query));
   for (uint32_t i = 0; i < S->b->nbstats; ++i) st->traversed_edges += bs[4ull * i + 1];
   return RLC_OK;
 }

@@ -369,3 +369,40 @@ def test_exact_vote_chunked_hub_majority(ctx, weighted):
         with ctx.upload(g) as dg:
             r = dg.run(rlc.options(mode=mode, slots=0, max_iterations=1), init=lab0)
         assert r.membership[:K].tolist() == [want] * K, (mode, r.membership[:K])
+
+
+def test_concurrent_runs_on_separate_graphs(ctx):
+    # graphs of one context run from several host threads at once (each graph has its own streams,
+    # and its captured run is captured thread-locally): every result equals the run alone
+    import threading
+
+    graphs = [cfg1(), random_weighted(20000, 160000, seed=11), with_hubs(20000, hubs=10, hub_degree=5000,
+                                                                         base_degree=12, seed=3)]
+    modes = [rlc.MODE_SYNC, rlc.MODE_SYNC, rlc.MODE_SYNC]
+    alone = []
+    for g, m in zip(graphs, modes):
+        with ctx.upload(g) as dg:
+            alone.append(dg.run(rlc.options(mode=m)).membership)
+    dgs = [ctx.upload(g) for g in graphs]
+    out = [[] for _ in graphs]
+    errors = []
+
+    def work(k):
+        try:
+            for _ in range(4):
+                out[k].append(dgs[k].run(rlc.options(mode=modes[k])).membership)
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(len(graphs))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for dg in dgs:
+        dg.close()
+    assert not errors, errors
+    for k in range(len(graphs)):
+        assert len(out[k]) == 4
+        for lab in out[k]:
+            assert np.array_equal(lab, alone[k])
