@@ -55,7 +55,7 @@ def load_traffic(args):
     """HBM bytes per move-phase iteration measured by rocprofv3 counters on this workload (profiles/)."""
     path = os.path.join(ROOT, "profiles", "rmat24_mg8_async_traffic.json")
     if (args.workload != "rmat" or args.scale != 24 or args.unit or args.slots != 8 or args.group_degree != "0,0,0,0"
-            or args.sweeps != 0):
+            or args.sweeps != 0 or args.always_mark):
         return None
     try:
         with open(path) as f:
@@ -310,6 +310,8 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--always-mark", action="store_true",
+                    help="every moved vertex marks its neighbours in every iteration (no blanket iterations)")
     ap.add_argument("--workload", choices=["rmat", "grid", "sbm", "batch", "serve"], default="rmat",
                     help="rmat: R-MAT 24 async (default); grid: 10^8-vertex road grid; sbm: the 65,536-vertex SBM "
                          "in the deterministic synchronous mode; batch: 4096-graph arena; serve: continuous batching")
@@ -364,7 +366,8 @@ def main():
     log(f"graph: {name}: n={g.n} m={g.m} average degree {g.m / g.n:.2f} max_degree={int(g.degrees().max())} "
         f"({time.time() - t0:.1f}s to generate)")
     opts = rlc.options(mode=rlc.MODE_SYNC if sync else rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
-                       group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps)
+                       group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps,
+                       always_mark=1 if args.always_mark else 0)
     ctx = rlc.Context(0)
     t0 = time.time()
     dg = ctx.upload(g)
@@ -388,6 +391,7 @@ def main():
     q, ncom = dg.modularity()
     per_iter = [
         {"l": i, "active": int(st.active[i]), "edges": int(st.edges[i]), "changed": int(st.changed[i]),
+         "changed_edges": int(st.changed_edges[i]), "blanket": bool(st.blanket_iterations >> i & 1),
          "move_ms": round(st.iter_ms[i], 4)}
         for i in range(min(st.iterations, rlc.MAX_ITER_STATS))
     ]

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define RLC_ABI_VERSION 1
+#define RLC_ABI_VERSION 2
 #define RLC_MAX_ITER_STATS 64
 #define RLC_MAX_SWEEPS 64
 #define RLC_EMPTY_LABEL 0xFFFFFFFFu /* reserved: never a valid label */
@@ -99,7 +99,14 @@ typedef struct rlc_options {
                              reference GPU kernels do (RESCAN=false, rakLowmemCuda.hxx:469, :624);
                              0 = rescan whenever the sketch cannot certify the choice (the CPU
                              reference's RESCAN=true result) [0]. SYNC: must be 0 */
-  int32_t reserved[3];    /* must be zero */
+  int32_t always_mark;    /* ASYNC, MG8: 1 = every moved vertex flags its neighbours as it moves, as the
+                             reference does (rakMarkNeighborsCudU, rakCuda.hxx:95-103); 0 = blanket
+                             iterations (the first, and each after one whose moved vertices held at least a
+                             quarter of the edge positions) mark nothing while they run; at their end they
+                             flag every vertex if their own moved vertices hold a quarter of the edge
+                             positions, else the moved vertices mark their neighbours then [0].
+                             SYNC, BM, EXACT and batches: ignored */
+  int32_t reserved[2];    /* must be zero */
 } rlc_options;
 
 typedef struct rlc_run_stats {
@@ -117,6 +124,10 @@ typedef struct rlc_run_stats {
   uint64_t changed[RLC_MAX_ITER_STATS];       /* C_l: vertices that changed label */
   uint64_t changed_edges[RLC_MAX_ITER_STATS]; /* X_l: sum of degrees of changed vertices */
   float iter_ms[RLC_MAX_ITER_STATS];          /* move-launch time (RLC_TIMING_PER_ITERATION only) */
+  uint64_t blanket_iterations; /* bit l: iteration l marked no neighbours and flagged every vertex at its end
+                                  (rlc_options::always_mark) */
+  uint64_t blanket_edges;      /* the fill threshold of those iterations: edge positions held by moved
+                                  vertices (0 when not in use) */
 } rlc_run_stats;
 
 typedef struct rlc_context rlc_context;

@@ -158,6 +158,34 @@ __device__ __forceinline__ void mark_neighbours(const DevGraph &g, uint8_t *aff,
   }
 }
 
+// The marks of a blanket iteration, made at its end: flag 0 -> 1 (marked), 2 -> 3 (moved in the
+// iteration and marked). A byte changes at most once, and never between the moved (>= 2) and the
+// unmoved values, so concurrent markers of one vertex agree.
+__device__ __forceinline__ void mark_neighbours_deferred(const DevGraph &g, uint8_t *aff, uint32_t u, uint64_t i,
+                                                         uint64_t e, uint64_t stride) {
+  for (; i < e; i += stride * kBatch) {
+    uint32_t vv[kBatch];
+    uint8_t fl[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint64_t j = i + (uint64_t)k * stride;
+      vv[k] = j < e ? g.edg[j] : u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) fl[k] = aff[vv[k]];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (vv[k] != u && (fl[k] == 0 || fl[k] == 2)) aff[vv[k]] = fl[k] + 1;
+  }
+}
+
+// A blanket iteration (asynchronous MG8) marks no neighbours while it runs; processed vertices
+// keep flag 2 if they moved. At its end, if its moved vertices hold at least `thr` edge positions,
+// every vertex is flagged for the next iteration (marking would flag nearly all of them, at the
+// cost of a second pass over those positions); otherwise the moved vertices mark their
+// neighbours then. thr is the graph's own (LoopParams::blanket_edges).
+__device__ __forceinline__ bool blanket_fills(const DevCounters *c, uint64_t thr) { return c->X >= thr; }
+
 __device__ __forceinline__ uint32_t range_of(const RangeSet &rs, uint64_t i) {
   uint32_t lo = 0, hi = rs.count;  // acc[lo] <= i < acc[hi]
   while (hi - lo > 1) {

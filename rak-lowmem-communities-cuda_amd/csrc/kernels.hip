@@ -50,6 +50,7 @@ template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
 __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, RangeSet rs, int pickstep) {
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
+  const bool blanket = !SYNC && MG && s.ctr->blanket != 0;
   const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
   const bool store = store_labels<RESCAN>(iter);
   unsigned long long sa = 0, se = 0, sc = 0, sx = 0;
@@ -104,10 +105,10 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
         wb = mw;
       }
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
-      s.aff[u] = 0;
+      s.aff[u] = blanket && moved ? 2 : 0;
       if (!SYNC && moved) {
         s.lab[u] = cb;
-        mark_neighbours(g, s.aff, u, b, e, 1);
+        if (!blanket) mark_neighbours(g, s.aff, u, b, e, 1);
       }
     }
     sa += act ? 1 : 0;
@@ -147,6 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
   constexpr int VPB = kBlock / P;     // vertices per block round
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
+  const bool blanket = !SYNC && s.ctr->blanket != 0;
   const bool pickless = pickstep > 0 && (iter % (uint32_t)pickstep) == 0;
   const int lane = lane_id();
   const int grp = (int)threadIdx.x / P;  // vertex slot in the block
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
       }
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       if (gl == 0) {
-        s.aff[u] = 0;
+        s.aff[u] = blanket && moved ? 2 : 0;
         sa += 1;
         se += deg;
       }
@@ -319,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
           sc += 1;
           sx += deg;
         }
-        mark_neighbours(g, s.aff, u, b + gl, e, P);
+        if (!blanket) mark_neighbours(g, s.aff, u, b + gl, e, P);
       }
     }
     if (SYNC) {
@@ -832,8 +834,12 @@ __global__ __launch_bounds__(kBlock) void k_chunk_choose(DevGraph g, DevState s,
   flush_stats(s.ctr, sa, se, sc, sx);
 }
 
-__global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, ChunkPlan cp) {
+// deferred = 0: right after the chunks' choice (not in a blanket iteration); 1: at the end of a
+// blanket iteration that does not flag every vertex
+__global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, ChunkPlan cp, int deferred,
+                                                       uint64_t thr) {
   if (s.ctr->done) return;
+  if (deferred ? !s.ctr->blanket || blanket_fills(s.ctr, thr) : s.ctr->blanket != 0) return;
   const int lane = lane_id();
   const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
   const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
@@ -844,13 +850,73 @@ __global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, C
     const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
     const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
     const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
-    mark_neighbours(g, s.aff, u, b + lane, e, 64);
+    if (deferred) mark_neighbours_deferred(g, s.aff, u, b + lane, e, 64);
+    else mark_neighbours(g, s.aff, u, b + lane, e, 64);
   }
 }
 
 // ---- end of an iteration: convergence test on the device ------------------------------------
-__global__ void k_finish(DevCounters *ctr, uint64_t n, double tolerance, int max_iterations, int pickstep,
-                         int sync) {
+
+// End of a blanket iteration (see blanket_fills): every flag set to 1, 16 per store ...
+__global__ __launch_bounds__(kBlock) void k_blanket_fill(uint8_t *aff, uint64_t n, uint64_t thr,
+                                                         const DevCounters *ctr) {
+  if (ctr->done || !ctr->blanket || !blanket_fills(ctr, thr)) return;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x, T = (uint64_t)gridDim.x * kBlock;
+  const uint64_t q = n / 16;
+  for (uint64_t i = t; i < q; i += T) reinterpret_cast<uint4 *>(aff)[i] = make_uint4(0x01010101u, 0x01010101u,
+                                                                                       0x01010101u, 0x01010101u);
+  for (uint64_t u = q * 16 + t; u < n; u += T) aff[u] = 1;
+}
+
+// ... or the moved vertices (flag >= 2) mark their neighbours: one wave per 64 consecutive
+// vertices, the lane's own edges up to kApplyLaneDegree, larger ones by the whole wave. Chunked
+// hubs kept flag 0 (their moves are in the chunk plan) and mark through k_chunk_mark.
+__global__ __launch_bounds__(kBlock) void k_deferred_mark(DevGraph g, uint8_t *aff, uint64_t thr,
+                                                          const DevCounters *ctr) {
+  if (ctr->done || !ctr->blanket || blanket_fills(ctr, thr)) return;
+  const int lane = lane_id();
+  const uint64_t wave_global = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
+  for (uint64_t base = wave_global * 64; base < g.n; base += nwaves * 64) {
+    const uint64_t p = base + (uint64_t)lane;
+    const bool moved = p < g.n && aff[p] >= 2;
+    const uint32_t u = (uint32_t)p;
+    uint64_t b = 0, e = 0;
+    if (moved) {
+      b = g.off[p];
+      e = g.off[p + 1];
+      if (e - b <= kApplyLaneDegree) mark_neighbours_deferred(g, aff, u, b, e, 1);
+    }
+    unsigned long long big = __ballot(moved && e - b > kApplyLaneDegree);
+    while (big) {
+      const int j = __ffsll((long long)big) - 1;
+      big &= big - 1;
+      const uint32_t uj = __shfl(u, j);
+      const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
+      mark_neighbours_deferred(g, aff, uj, bj + lane, ej, 64);
+    }
+  }
+}
+
+// After the deferred marks: flag = marked (2 -> 0, 3 -> 1), 16 flags per load.
+__global__ __launch_bounds__(kBlock) void k_deferred_clear(uint8_t *aff, uint64_t n, uint64_t thr,
+                                                           const DevCounters *ctr) {
+  if (ctr->done || !ctr->blanket || blanket_fills(ctr, thr)) return;
+  const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x, T = (uint64_t)gridDim.x * kBlock;
+  const uint64_t q = n / 16;
+  uint4 *a4 = reinterpret_cast<uint4 *>(aff);
+  for (uint64_t i = t; i < q; i += T) {
+    uint4 x = a4[i];
+    x.x &= 0x01010101u;
+    x.y &= 0x01010101u;
+    x.z &= 0x01010101u;
+    x.w &= 0x01010101u;
+    a4[i] = x;
+  }
+  for (uint64_t u = q * 16 + t; u < n; u += T) aff[u] &= 1;
+}
+__global__ void k_finish(DevCounters *ctr, uint64_t n, uint64_t thr, double tolerance, int max_iterations,
+                         int pickstep, int sync, int blanket_rule) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   if (ctr->done) return;
   const unsigned int l = ctr->iter;
@@ -866,6 +932,9 @@ __global__ void k_finish(DevCounters *ctr, uint64_t n, double tolerance, int max
   ctr->last_changed = C;
   // rakLowmemLoopCuU: ++l; if (!PICKLESS && double(n)/N <= E) break;  (rakLowmemCuda.hxx:486-487)
   if ((!pickless && double(C) / double(n) <= tolerance) || (int)(l + 1) >= max_iterations) ctr->done = 1;
+  if (ctr->blanket && l < 64) (blanket_fills(ctr, thr) ? ctr->blanket_mask : ctr->deferred_mask) |= 1ull << l;
+  // a blanket iteration follows one whose moved vertices held thr edge positions
+  ctr->blanket = blanket_rule && blanket_fills(ctr, thr) ? 1u : 0u;
   ctr->A = ctr->E = ctr->C = ctr->X = 0;
   ctr->nmoves = 0;
 }
@@ -876,7 +945,8 @@ __global__ __launch_bounds__(kBlock) void k_init_labels(uint32_t *lab, const uin
     lab[p] = init ? init[p] : perm[p];
 }
 
-__global__ __launch_bounds__(kBlock) void k_mark_all(uint8_t *aff, uint64_t n, DevCounters *ctr, int max_iterations) {
+__global__ __launch_bounds__(kBlock) void k_mark_all(uint8_t *aff, uint64_t n, DevCounters *ctr, int max_iterations,
+                                                     unsigned int blanket0) {
   const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   for (uint64_t u = t; u < n; u += (uint64_t)gridDim.x * kBlock) aff[u] = 1;
   if (t == 0) {
@@ -885,6 +955,9 @@ __global__ __launch_bounds__(kBlock) void k_mark_all(uint8_t *aff, uint64_t n, D
     ctr->iter = 0;
     ctr->last_changed = 0;
     ctr->done = max_iterations <= 0 ? 1u : 0u;
+    ctr->blanket = blanket0;
+    ctr->blanket_mask = 0;
+    ctr->deferred_mask = 0;
   }
   // per-iteration history: zeroed by the grid
   unsigned long long *h = &ctr->hist[0][0];
@@ -1033,8 +1106,9 @@ void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *in
   k_init_labels<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.lab, init, perm, g.n);
 }
 
-void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, hipStream_t st) {
-  k_mark_all<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.aff, g.n, s.ctr, max_iterations);
+void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, bool blanket0, hipStream_t st) {
+  k_mark_all<<<grid_for(g.n, kBlock, kGridCap), kBlock, 0, st>>>(s.aff, g.n, s.ctr, max_iterations,
+                                                                  blanket0 ? 1u : 0u);
 }
 
 template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
@@ -1047,7 +1121,7 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     k_chunk_merge<<<gv, kBlock, 0, st>>>(g, s, sc.plan, RESCAN ? 1 : 0);
     if (RESCAN) k_chunk_rescan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
     k_chunk_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
-    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan, 0, 0);
   }
   auto launch = [&](const MoveLaunch &L, hipStream_t q) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
@@ -1122,7 +1196,7 @@ static void launch_ht(const DevGraph &g, const DevState &s, const Schedule &sc, 
     k_htc_clear<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
     k_htc_insert<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
     k_htc_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
-    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan);
+    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan, 0, 0);
   }
   for (const MoveLaunch &L : sc.launches) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
@@ -1152,9 +1226,20 @@ void launch_iteration(const DevGraph &g, const DevState &s, const Schedule &cl, 
     if (p.mg) launch_moves_w<false, true>(g, s, cl, p, st, fk);
     else launch_moves_w<false, false>(g, s, cl, p, st, fk);
   }
+  const bool blanket = p.blanket && !p.sync && !p.exact && p.mg;
+  if (blanket) {  // the end of a blanket iteration: each kernel returns at once unless its case holds
+    const unsigned gq = grid_for((g.n + 15) / 16, kBlock, kGridCap);
+    const uint64_t thr = p.blanket_edges;
+    k_blanket_fill<<<gq, kBlock, 0, st>>>(s.aff, g.n, thr, s.ctr);
+    if (cl.chunked && cl.plan.nchunks > 0)
+      k_chunk_mark<<<grid_for(cl.plan.nchunks, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s, cl.plan, 1, thr);
+    k_deferred_mark<<<grid_for((g.n + 63) / 64, kBlock / 64, kGridCap), kBlock, 0, st>>>(g, s.aff, thr, s.ctr);
+    k_deferred_clear<<<gq, kBlock, 0, st>>>(s.aff, g.n, thr, s.ctr);
+  }
   if (ev_move1) (void)hipEventRecord(ev_move1, st);
   if (p.sync) k_apply<<<grid_for(g.n, kBlock, 4096), kBlock, 0, st>>>(g, s);
-  k_finish<<<1, 64, 0, st>>>(s.ctr, g.n, p.tolerance, p.max_iterations, p.pickstep, p.sync ? 1 : 0);
+  k_finish<<<1, 64, 0, st>>>(s.ctr, g.n, p.blanket_edges, p.tolerance, p.max_iterations, p.pickstep,
+                             p.sync ? 1 : 0, blanket ? 1 : 0);
 }
 
 namespace {

@@ -24,7 +24,11 @@ struct DevCounters {
   unsigned long long nmoves;      // synchronous mode: length of the change list
   unsigned int iter;              // iterations completed
   unsigned int done;              // set once converged or at the iteration cap
+  unsigned int blanket;           // asynchronous sketches: this iteration marks no neighbours, and every
+                                  // vertex is flagged at its end (k_blanket_fill)
   unsigned long long last_changed;
+  unsigned long long blanket_mask;   // bit l: iteration l was a blanket iteration that flagged every vertex
+  unsigned long long deferred_mask;  // bit l: iteration l was a blanket iteration that marked at its end
   unsigned long long hist[RLC_MAX_ITER_STATS][4];
 };
 
@@ -105,13 +109,20 @@ struct LoopParams {
   double tolerance;
   int max_iterations;
   bool exact = false;  // the exact hashtable vote (rakStaticCuda) instead of a sketch
+  // asynchronous MG8: blanket iterations (the first, and any after one whose moved vertices held
+  // blanket_edges edge positions) mark no neighbours while they run; at their end they flag every
+  // vertex if their own moved vertices hold blanket_edges positions, else mark the moved vertices'
+  // neighbours then
+  bool blanket = false;
+  uint64_t blanket_edges = 0;
 };
 
 // flags = 1; counters reset (done = max_iterations <= 0)
 // lab[p] = init ? init[p] : perm[p] (label values are original vertex ids)
 void launch_init_labels(const DevGraph &g, const DevState &s, const uint32_t *init, const uint32_t *perm,
                         hipStream_t st);
-void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, hipStream_t st);
+// blanket0: the first iteration is a blanket iteration
+void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, bool blanket0, hipStream_t st);
 // One iteration: move launches for each non-empty class, the apply pass (synchronous mode) and
 // the finish step. ev_move0/ev_move1 (may be null) bracket the move launches.
 // fk (may be null): the degree classes of each sub-sweep run concurrently on its streams.

@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librlc.so")
 
 MAX_ITER_STATS = 64
+ABI_VERSION = 2  # RLC_ABI_VERSION of the rlc.h these structures mirror
 EMPTY_LABEL = 0xFFFFFFFF
 MODE_ASYNC, MODE_SYNC = 0, 1
 SKETCH_BM, SKETCH_MG8 = 1, 8
@@ -37,7 +38,8 @@ class Options(C.Structure):
         ("repeat", C.c_int32), ("tolerance", C.c_double), ("max_iterations", C.c_int32),
         ("slots", C.c_int32), ("mode", C.c_int32), ("pickstep", C.c_int32),
         ("merge_degree", C.c_int32), ("merge_parts", C.c_int32), ("group_degree", C.c_int32 * 4),
-        ("timing", C.c_int32), ("sweeps", C.c_int32), ("no_rescan", C.c_int32), ("reserved", C.c_int32 * 3),
+        ("timing", C.c_int32), ("sweeps", C.c_int32), ("no_rescan", C.c_int32), ("always_mark", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 
@@ -48,7 +50,8 @@ class RunStats(C.Structure):
         ("traversed_edges", C.c_uint64), ("algorithmic_bytes", C.c_uint64),
         ("active", C.c_uint64 * MAX_ITER_STATS), ("edges", C.c_uint64 * MAX_ITER_STATS),
         ("changed", C.c_uint64 * MAX_ITER_STATS), ("changed_edges", C.c_uint64 * MAX_ITER_STATS),
-        ("iter_ms", C.c_float * MAX_ITER_STATS),
+        ("iter_ms", C.c_float * MAX_ITER_STATS), ("blanket_iterations", C.c_uint64),
+        ("blanket_edges", C.c_uint64),
     ]
 
 
@@ -177,6 +180,8 @@ def lib() -> C.CDLL:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        if L.rlc_abi_version() != ABI_VERSION:  # the structures below mirror this version of rlc.h
+            raise RlcError(10, f"{LIB_PATH} has ABI version {L.rlc_abi_version()}, this binding {ABI_VERSION}")
         _lib = L
     return _lib
 

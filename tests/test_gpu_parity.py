@@ -206,6 +206,88 @@ def test_async_hub_graph_chunks(ctx, slots):
         print(f"hubs async slots={slots} groups={gd}: Q={q:.5f} iters={r.iterations} communities={len(set(lab))}")
 
 
+def blanket_bits_expected(X, T, L):
+    """Iterations that flag every vertex at their end: blanket iterations (the first, and each after
+    one whose moved vertices held >= T edge positions) whose own moved vertices hold >= T."""
+    bits = 0
+    for l in range(L):
+        blanket = l == 0 or X[l - 1] >= T
+        if blanket and X[l] >= T:
+            bits |= 1 << l
+    return bits
+
+
+def blanket_threshold_np(g):
+    """The fill threshold from the degree histogram: the smallest share x of moved edge positions
+    with 2 U(x) <= x, U(x) = sum_v d_v (1 - x)^d_v / m (positions left unflagged by marking when
+    moves are spread independently), as edge positions."""
+    d = g.degrees().astype(np.float64)
+    d = d[d > 0]
+    m = d.sum()
+    xs = np.linspace(0.0, 1.0, 200001)[1:]
+    u = np.array([(d * np.exp(d * np.log1p(-x))).sum() / m if x < 1 else 0.0 for x in xs[::100]])
+    # coarse grid first, then refine around the first x that satisfies the inequality
+    k = int(np.argmax(2 * u <= xs[::100]))
+    lo = xs[::100][k - 1] if k > 0 else 0.0
+    fine = xs[(xs >= lo) & (xs <= xs[::100][k])]
+    uf = np.array([(d * np.exp(d * np.log1p(-x))).sum() / m if x < 1 else 0.0 for x in fine])
+    return float(fine[int(np.argmax(2 * uf <= fine))]) * m
+
+
+@pytest.mark.parametrize("slots", [8, 1])
+def test_async_blanket_marking(ctx, slots):
+    # MG8: blanket iterations mark no neighbours while they run, and at their end flag every vertex
+    # or mark the moved vertices' neighbours; always_mark = 1 (and BM) mark as the reference does
+    graphs = [("cfg1", cfg1()), ("rmat15", rlc.gen_rmat(15, 16, wmin=0.5, wmax=1.5, seed=2)),
+              ("grid", rlc.gen_grid(120, 150, 0.1, 0.02, 3)),
+              ("hubs", with_hubs(20000, hubs=40, hub_degree=9000, base_degree=10, seed=5))]
+    for name, g in graphs:
+        q_ref = O.modularity(g, O.lowmem_omp(g, slots == 8)[0])
+        with ctx.upload(g) as dg:
+            qs = {}
+            for am in (0, 1):
+                r = dg.run(rlc.options(slots=slots, always_mark=am))
+                st = r.stats
+                L = r.iterations
+                bits = st.blanket_iterations
+                X = [int(st.changed_edges[l]) for l in range(L)]
+                A = [int(st.active[l]) for l in range(L)]
+                T = int(st.blanket_edges)
+                if am == 1 or slots != 8:
+                    assert bits == 0 and T == 0
+                else:
+                    # the threshold against a grid search of the same inequality (step 5e-6 of m)
+                    assert abs(T - blanket_threshold_np(g)) <= 1e-5 * g.m + 2, (name, T)
+                    assert 0 < T <= g.m
+                    assert bits == blanket_bits_expected(X, T, L), (name, X, T, bits)
+                for l in range(1, L):
+                    if bits >> (l - 1) & 1:
+                        assert A[l] == g.n  # every vertex flagged after a filling iteration
+                qs[am], _ = dg.modularity()
+                conv = r.membership
+                print(f"{name} slots={slots} always_mark={am}: Q={qs[am]:.5f} ref={q_ref:.5f} iters={L} "
+                      f"fills={bits:b} T={T} X={X} A={A}")
+            # from converged labels few vertices move: the first iteration marks at its end
+            r = dg.run(rlc.options(slots=slots), init=conv)
+            st = r.stats
+            X0 = int(st.changed_edges[0])
+            assert X0 < st.blanket_edges and st.blanket_iterations & 1 == 0 or slots != 8
+            if r.iterations > 1:
+                assert int(st.active[1]) < g.n
+            q_conv, _ = dg.modularity()
+            print(f"{name} slots={slots} from converged labels: iters={r.iterations} X0={X0} "
+                  f"A1={int(st.active[1]) if r.iterations > 1 else None} Q={q_conv:.5f}")
+            assert q_conv >= qs[1] - max(0.01 * abs(qs[1]), 1e-3)
+            # the deterministic mode never uses it
+            assert dg.run(rlc.options(slots=slots, mode=rlc.MODE_SYNC)).stats.blanket_iterations == 0
+            with pytest.raises(rlc.RlcError):
+                dg.run(rlc.options(slots=slots, always_mark=2))
+        if slots == 8 and name in ("cfg1", "grid"):
+            assert qs[0] >= q_ref * 0.99 and qs[1] >= q_ref * 0.99
+        if name == "rmat15":  # no community structure (Q ~1e-4): the absolute gate
+            assert qs[0] >= q_ref - max(0.01 * abs(q_ref), 1e-3)
+
+
 def test_async_bm_runs(ctx):
     g = cfg1()
     r, q = async_quality(ctx, g, slots=1)
