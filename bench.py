@@ -418,7 +418,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "move phase (lane/wave/workgroup move launches of each iteration)",
+            "kernel": "move phase (the lane/wave/workgroup move launches of each iteration and its end-of-iteration flag passes)",
             "algorithmic_bytes_per_step": total_bytes / args.steps, "move_ms_per_step": move_ms / args.steps,
         },
     }
