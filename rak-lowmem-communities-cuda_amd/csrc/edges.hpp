@@ -158,11 +158,19 @@ __device__ __forceinline__ void mark_neighbours(const DevGraph &g, uint8_t *aff,
   }
 }
 
-// The marks of a blanket iteration, made at its end: flag 0 -> 1 (marked), 2 -> 3 (moved in the
-// iteration and marked). A byte changes at most once, and never between the moved (>= 2) and the
-// unmoved values, so concurrent markers of one vertex agree.
+// Flags in a blanket iteration: a processed vertex leaves (phase + 1) << 2 | moved << 1, where
+// phase orders the launches of the iteration (0 the chunked hubs, s + 1 sub-sweep s; at most 8
+// sub-sweeps, so the byte holds it); bit 0 is the mark made at the iteration's end.
+__device__ __forceinline__ uint8_t processed_flag(int phase, bool moved) {
+  return (uint8_t)(((phase + 1) << 2) | (moved ? 2 : 0));
+}
+
+// The marks of a blanket iteration, made at its end by a moved vertex u whose flag class is cu
+// (flag >> 2). A neighbour processed in a later phase already read u's new label and is left alone;
+// one processed before or beside u, or not at all, is marked (bit 0 set). Only bit 0 ever changes,
+// and every marker of a vertex writes the same byte, so concurrent markers agree.
 __device__ __forceinline__ void mark_neighbours_deferred(const DevGraph &g, uint8_t *aff, uint32_t u, uint64_t i,
-                                                         uint64_t e, uint64_t stride) {
+                                                         uint64_t e, uint64_t stride, uint32_t cu) {
   for (; i < e; i += stride * kBatch) {
     uint32_t vv[kBatch];
     uint8_t fl[kBatch];
@@ -174,13 +182,15 @@ __device__ __forceinline__ void mark_neighbours_deferred(const DevGraph &g, uint
 #pragma unroll
     for (int k = 0; k < kBatch; ++k) fl[k] = aff[vv[k]];
 #pragma unroll
-    for (int k = 0; k < kBatch; ++k)
-      if (vv[k] != u && (fl[k] == 0 || fl[k] == 2)) aff[vv[k]] = fl[k] + 1;
+    for (int k = 0; k < kBatch; ++k) {
+      const uint32_t cv = (uint32_t)fl[k] >> 2;
+      if (vv[k] != u && !(fl[k] & 1) && (cv == 0 || cv <= cu)) aff[vv[k]] = (uint8_t)(fl[k] | 1);
+    }
   }
 }
 
 // A blanket iteration (asynchronous MG8) marks no neighbours while it runs; processed vertices
-// keep flag 2 if they moved. At its end, if its moved vertices hold at least `thr` edge positions,
+// keep their processed_flag. At its end, if its moved vertices hold at least `thr` edge positions,
 // every vertex is flagged for the next iteration (marking would flag nearly all of them, at the
 // cost of a second pass over those positions); otherwise the moved vertices mark their
 // neighbours then. thr is the graph's own (LoopParams::blanket_edges).

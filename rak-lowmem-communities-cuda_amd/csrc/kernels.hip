@@ -47,7 +47,8 @@ __device__ __forceinline__ bool store_labels(uint32_t iter) {
 // ---- lane per vertex: one sequential sketch per lane ----------------------------------------
 // Items are 64-position chunks of the ranges of `rs`; one chunk per wave.
 template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
-__global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, RangeSet rs, int pickstep) {
+__global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, RangeSet rs, int pickstep,
+                                                      int phase) {
   if (s.ctr->done) return;
   const uint32_t iter = s.ctr->iter;
   const bool blanket = !SYNC && MG && s.ctr->blanket != 0;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
         wb = mw;
       }
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
-      s.aff[u] = blanket && moved ? 2 : 0;
+      s.aff[u] = blanket ? processed_flag(phase, moved) : 0;
       if (!SYNC && moved) {
         s.lab[u] = cb;
         if (!blanket) mark_neighbours(g, s.aff, u, b, e, 1);
@@ -142,7 +143,8 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
 // ---- P lanes per vertex: groups of P = 4, 16, 64 lanes inside a wave, or P = 256 (4 waves) --
 // Items are vertices of the ranges of `rs`; part p of a vertex scans positions p, p+P, ...
 template <bool SYNC, bool MG, bool UNIT, int P, bool RESCAN>
-__global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, RangeSet rs, int pickstep) {
+__global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, RangeSet rs, int pickstep,
+                                                       int phase) {
   constexpr int W = P < 64 ? P : 64;  // lanes of a group inside one wave
   constexpr int GW = P / W;           // waves per vertex
   constexpr int VPB = kBlock / P;     // vertices per block round
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_move_group(DevGraph g, DevState s, R
       }
       moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       if (gl == 0) {
-        s.aff[u] = blanket && moved ? 2 : 0;
+        s.aff[u] = blanket ? processed_flag(phase, moved) : 0;
         sa += 1;
         se += deg;
       }
@@ -822,7 +824,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_choose(DevGraph g, DevState s,
       const uint32_t d = s.lab[u];
       const bool moved = wb > 0.0f && cb != d && !(pickless && cb > d);
       const uint64_t deg = g.off[(uint64_t)u + 1] - g.off[u];
-      s.aff[u] = 0;
+      s.aff[u] = s.ctr->blanket ? processed_flag(0, false) : 0;  // the moves are in the plan
       if (moved) s.lab[u] = cb;
       cp.moved[slot] = moved ? 1u : 0u;
       sa += 1;
@@ -850,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, C
     const uint64_t vb = g.off[u], ve = g.off[(uint64_t)u + 1];
     const uint64_t b = vb + (uint64_t)ci * cp.chunk_edges;
     const uint64_t e = b + cp.chunk_edges < ve ? b + cp.chunk_edges : ve;
-    if (deferred) mark_neighbours_deferred(g, s.aff, u, b + lane, e, 64);
+    if (deferred) mark_neighbours_deferred(g, s.aff, u, b + lane, e, 64, 1);
     else mark_neighbours(g, s.aff, u, b + lane, e, 64);
   }
 }
@@ -868,9 +870,9 @@ __global__ __launch_bounds__(kBlock) void k_blanket_fill(uint8_t *aff, uint64_t 
   for (uint64_t u = q * 16 + t; u < n; u += T) aff[u] = 1;
 }
 
-// ... or the moved vertices (flag >= 2) mark their neighbours: one wave per 64 consecutive
+// ... or the moved vertices (flag bit 1) mark their neighbours: one wave per 64 consecutive
 // vertices, the lane's own edges up to kApplyLaneDegree, larger ones by the whole wave. Chunked
-// hubs kept flag 0 (their moves are in the chunk plan) and mark through k_chunk_mark.
+// hubs keep bit 1 clear (their moves are in the chunk plan) and mark through k_chunk_mark.
 __global__ __launch_bounds__(kBlock) void k_deferred_mark(DevGraph g, uint8_t *aff, uint64_t thr,
                                                           const DevCounters *ctr) {
   if (ctr->done || !ctr->blanket || blanket_fills(ctr, thr)) return;
@@ -879,26 +881,27 @@ __global__ __launch_bounds__(kBlock) void k_deferred_mark(DevGraph g, uint8_t *a
   const uint64_t nwaves = ((uint64_t)gridDim.x * kBlock) >> 6;
   for (uint64_t base = wave_global * 64; base < g.n; base += nwaves * 64) {
     const uint64_t p = base + (uint64_t)lane;
-    const bool moved = p < g.n && aff[p] >= 2;
-    const uint32_t u = (uint32_t)p;
+    const uint32_t fl = p < g.n ? aff[p] : 0u;
+    const bool moved = (fl & 2u) != 0;
+    const uint32_t u = (uint32_t)p, cu = fl >> 2;
     uint64_t b = 0, e = 0;
     if (moved) {
       b = g.off[p];
       e = g.off[p + 1];
-      if (e - b <= kApplyLaneDegree) mark_neighbours_deferred(g, aff, u, b, e, 1);
+      if (e - b <= kApplyLaneDegree) mark_neighbours_deferred(g, aff, u, b, e, 1, cu);
     }
     unsigned long long big = __ballot(moved && e - b > kApplyLaneDegree);
     while (big) {
       const int j = __ffsll((long long)big) - 1;
       big &= big - 1;
-      const uint32_t uj = __shfl(u, j);
+      const uint32_t uj = __shfl(u, j), cj = __shfl(cu, j);
       const uint64_t bj = __shfl(b, j), ej = __shfl(e, j);
-      mark_neighbours_deferred(g, aff, uj, bj + lane, ej, 64);
+      mark_neighbours_deferred(g, aff, uj, bj + lane, ej, 64, cj);
     }
   }
 }
 
-// After the deferred marks: flag = marked (2 -> 0, 3 -> 1), 16 flags per load.
+// After the deferred marks: flag = marked (bit 0), 16 flags per load.
 __global__ __launch_bounds__(kBlock) void k_deferred_clear(uint8_t *aff, uint64_t n, uint64_t thr,
                                                            const DevCounters *ctr) {
   if (ctr->done || !ctr->blanket || blanket_fills(ctr, thr)) return;
@@ -1125,26 +1128,27 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
   }
   auto launch = [&](const MoveLaunch &L, hipStream_t q) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
+    const int ph = L.group >= 0 ? L.group + 1 : 1;  // launch order inside the iteration (processed_flag)
     switch (L.lanes) {
       case 1:
         k_move_lane<SYNC, MG, UNIT, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
-                                                                                            p.pickstep);
+                                                                                            p.pickstep, ph);
         break;
       case 4:
         k_move_group<SYNC, MG, UNIT, 4, RESCAN><<<grid_for(items, kBlock / 4, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
-                                                                                               p.pickstep);
+                                                                                               p.pickstep, ph);
         break;
       case 16:
         k_move_group<SYNC, MG, UNIT, 16, RESCAN><<<grid_for(items, kBlock / 16, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
-                                                                                                 p.pickstep);
+                                                                                                 p.pickstep, ph);
         break;
       case 64:
         k_move_group<SYNC, MG, UNIT, 64, RESCAN><<<grid_for(items, kBlock / 64, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
-                                                                                                 p.pickstep);
+                                                                                                 p.pickstep, ph);
         break;
       default:
         k_move_group<SYNC, MG, UNIT, 256, RESCAN><<<grid_for(items, 1, kGridCap), kBlock, 0, q>>>(g, s, L.ranges,
-                                                                                        p.pickstep);
+                                                                                        p.pickstep, ph);
         break;
     }
   };
