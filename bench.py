@@ -13,10 +13,13 @@ sum of the degrees of the vertices it processes.
 
 Roofline: the move launches of each iteration are bracketed by HIP events on the graph's stream;
 `achieved` = algorithmic bytes of the move phase (per iteration: N flag bytes + A (8+4+1) +
-12 E + 4 C + 5 X) / measured move time, against the 8 TB/s HBM peak.
+12 E + 4 C + 5 X, where A, E, C, X are the active vertices, their edges, the moved vertices and
+their edges; an iteration that flags every vertex at its end writes N flag bytes instead of 5 X,
+one that marks at its end adds 3 N) / measured move time, against the 8 TB/s HBM peak.
 
 cpu_baseline: the oracle's restatement of rakLowmemStaticOmp<true,true,8> (pinned bit-exact to
-the compiled reference single-threaded) on the same graph, with all host cores.
+the compiled reference single-threaded) on the same graph, with all host cores; beside it the same
+method's first iteration on one thread, and GVE-LPA (rakStaticOmp restated) with all host cores.
 """
 import argparse
 import json
@@ -475,6 +478,17 @@ def main():
             "sample": f"{what} of {'MG8' if args.slots == 8 else 'BM'} (rakLowmemStaticOmp restated) on the same "
                       f"graph: {it} iterations, {trav} traversed edges, {tms / 1e3:.2f} s",
         }
+        # the same method on one thread (its first iteration), and GVE-LPA (rakStaticOmp restated,
+        # exact weighted vote) with all cores
+        _, it1, tms1, trav1 = oracle.lowmem_omp(g, args.slots == 8, threads=1, max_iterations=1)
+        line["cpu_baseline"]["single_thread"] = {
+            "value": trav1 / (tms1 * 1e-3), "cores": 1,
+            "sample": f"the first iteration on one thread: {trav1} traversed edges, {tms1 / 1e3:.2f} s"}
+        _, itg, tmsg, travg = oracle.rak_omp(g, threads=cores, max_iterations=20 if full else 1)
+        line["cpu_baseline"]["gve_lpa"] = {
+            "value": travg / (tmsg * 1e-3), "cores": cores,
+            "sample": f"{what} of GVE-LPA (rakStaticOmp restated): {itg} iterations, {travg} traversed edges, "
+                      f"{tmsg / 1e3:.2f} s"}
     print(json.dumps(line), flush=True)
 
 
