@@ -288,6 +288,37 @@ def test_async_blanket_marking(ctx, slots):
             assert qs[0] >= q_ref - max(0.01 * abs(q_ref), 1e-3)
 
 
+@pytest.mark.parametrize("variant", ["per-iteration", "one-sweep", "no-chunks"])
+def test_async_blanket_marking_schedules(ctx, variant):
+    # the blanket rule on the other schedules: host-driven iterations (no captured graph), a single
+    # sweep (one phase for every non-hub class), and no chunked hubs (the wave class marks them)
+    kw = {"per-iteration": dict(timing=rlc.TIMING_PER_ITERATION), "one-sweep": dict(sweeps=1),
+          "no-chunks": dict(group_degree=(32, 128, 512, 1 << 30))}[variant]
+    for name, g in [("cfg1", cfg1()), ("hubs", with_hubs(20000, hubs=40, hub_degree=9000, base_degree=10, seed=5))]:
+        with ctx.upload(g) as dg:
+            r = dg.run(rlc.options(**kw))
+            st = r.stats
+            L = r.iterations
+            X = [int(st.changed_edges[l]) for l in range(L)]
+            A = [int(st.active[l]) for l in range(L)]
+            T = int(st.blanket_edges)
+            assert st.blanket_iterations == blanket_bits_expected(X, T, L), (name, X, T)
+            for l in range(1, L):
+                if st.blanket_iterations >> (l - 1) & 1:
+                    assert A[l] == g.n
+            q, _ = dg.modularity()
+            r1 = dg.run(rlc.options(always_mark=1, **kw))
+            q1, _ = dg.modularity()
+            # from converged labels: the first iteration marks at its end
+            r2 = dg.run(rlc.options(**kw), init=r1.membership)
+            assert r2.stats.blanket_iterations & 1 == 0
+            q2, _ = dg.modularity()
+            print(f"{variant} {name}: Q={q:.5f} always_mark Q={q1:.5f} from converged Q={q2:.5f} iters={L} "
+                  f"fills={st.blanket_iterations:b}")
+            if name == "cfg1":  # run-to-run spread of the asynchronous mode on these schedules: ~1%
+                assert q >= q1 - 0.02 * abs(q1) and q2 >= q1 - 0.02 * abs(q1)
+
+
 def test_async_bm_runs(ctx):
     g = cfg1()
     r, q = async_quality(ctx, g, slots=1)
