@@ -28,6 +28,8 @@ namespace {
 
 constexpr int kDefaultGroupDegree[4] = {32, 128, 512, 4096};  // 4, 16, 64, 256 lanes per vertex
 constexpr int kMaxSegments = 8;
+// a blanket iteration's flag byte holds (phase + 1) << 2 with phase <= kMaxSegments (processed_flag)
+static_assert(((kMaxSegments + 1) << 2 | 3) < 256, "phase does not fit the flag byte");
 constexpr uint64_t kSegmentVertices = 4096;  // at least this many vertices per segment
 
 int segments_for(uint64_t n) {
