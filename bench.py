@@ -417,6 +417,11 @@ def main():
         "config": {
             "workload": f"{name}, {METHOD[args.slots]} {'sync' if sync else 'async'}, tol 0.05, max 20 iterations",
             "n": g.n, "m": g.m, "iterations": iters, "modularity": q, "communities": ncom,
+            # vertices processed include every vertex after an iteration that flagged all of them
+            # (asynchronous MG8, always_mark = 0): the time per step is the work-independent figure
+            "traversed_edges_per_step": total_edges / args.steps,
+            "fill_iterations_last_step": [i for i in range(min(st.iterations, rlc.MAX_ITER_STATS))
+                                          if st.blanket_iterations >> i & 1],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
