@@ -857,9 +857,9 @@ __global__ __launch_bounds__(kBlock) void k_chunk_mark(DevGraph g, DevState s, C
   }
 }
 
-// ---- end of an iteration: convergence test on the device ------------------------------------
+// ---- end of a blanket iteration (asynchronous MG8; see blanket_fills) --------------------------
 
-// End of a blanket iteration (see blanket_fills): every flag set to 1, 16 per store ...
+// Every flag set to 1, 16 per store ...
 __global__ __launch_bounds__(kBlock) void k_blanket_fill(uint8_t *aff, uint64_t n, uint64_t thr,
                                                          const DevCounters *ctr) {
   if (ctr->done || !ctr->blanket || !blanket_fills(ctr, thr)) return;
@@ -918,6 +918,8 @@ __global__ __launch_bounds__(kBlock) void k_deferred_clear(uint8_t *aff, uint64_
   }
   for (uint64_t u = q * 16 + t; u < n; u += T) aff[u] &= 1;
 }
+
+// ---- end of an iteration: convergence test on the device ------------------------------------
 __global__ void k_finish(DevCounters *ctr, uint64_t n, uint64_t thr, double tolerance, int max_iterations,
                          int pickstep, int sync, int blanket_rule) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
