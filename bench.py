@@ -58,7 +58,7 @@ def load_traffic(args):
     """HBM bytes per move-phase iteration measured by rocprofv3 counters on this workload (profiles/)."""
     path = os.path.join(ROOT, "profiles", "rmat24_mg8_async_traffic.json")
     if (args.workload != "rmat" or args.scale != 24 or args.unit or args.slots != 8 or args.group_degree != "0,0,0,0"
-            or args.sweeps != 0 or args.always_mark):
+            or args.sweeps != 0 or args.blanket):
         return None
     try:
         with open(path) as f:
@@ -313,8 +313,10 @@ def main():
     ap.add_argument("--group-degree", type=str, default="0,0,0,0",
                     help="degrees at which 4, 16, 64, 256 lanes scan a vertex (0,0,0,0 = library defaults)")
     ap.add_argument("--sweeps", type=int, default=0)
-    ap.add_argument("--always-mark", action="store_true",
-                    help="every moved vertex marks its neighbours in every iteration (no blanket iterations)")
+    ap.add_argument("--default-only", action="store_true",
+                    help="skip the opt-in blanket leg beside the default (profiling runs)")
+    ap.add_argument("--blanket", action="store_true",
+                    help="asynchronous MG8 with blanket iterations (rlc_options::blanket = 1) for the timed steps")
     ap.add_argument("--workload", choices=["rmat", "grid", "sbm", "batch", "serve"], default="rmat",
                     help="rmat: R-MAT 24 async (default); grid: 10^8-vertex road grid; sbm: the 65,536-vertex SBM "
                          "in the deterministic synchronous mode; batch: 4096-graph arena; serve: continuous batching")
@@ -370,7 +372,7 @@ def main():
         f"({time.time() - t0:.1f}s to generate)")
     opts = rlc.options(mode=rlc.MODE_SYNC if sync else rlc.MODE_ASYNC, slots=args.slots, timing=rlc.TIMING_PER_ITERATION,
                        group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps,
-                       always_mark=1 if args.always_mark else 0)
+                       blanket=1 if args.blanket else 0)
     ctx = rlc.Context(0)
     t0 = time.time()
     dg = ctx.upload(g)
@@ -417,8 +419,9 @@ def main():
         "config": {
             "workload": f"{name}, {METHOD[args.slots]} {'sync' if sync else 'async'}, tol 0.05, max 20 iterations",
             "n": g.n, "m": g.m, "iterations": iters, "modularity": q, "communities": ncom,
-            # vertices processed include every vertex after an iteration that flagged all of them
-            # (asynchronous MG8, always_mark = 0): the time per step is the work-independent figure
+            "pruning": "blanket iterations (opt-in)" if args.blanket else "the reference's neighbour marks",
+            # with blanket iterations every vertex is processed after an iteration that flagged all of
+            # them: the time per step is the work-independent figure
             "traversed_edges_per_step": total_edges / args.steps,
             "fill_iterations_last_step": [i for i in range(min(st.iterations, rlc.MAX_ITER_STATS))
                                           if st.blanket_iterations >> i & 1],
@@ -441,6 +444,23 @@ def main():
         "first_iteration_fraction": it0 / ceil if ceil > 0 else None,
     }
     log(f"gather probe: {probe_ms:.3f} ms per pass ({ceil:.3g} edges/s); first iteration {it0:.3g} edges/s")
+    if not sync and args.slots == 8 and not args.blanket and not args.default_only:
+        # beside the default: the same steps with the opt-in blanket iterations
+        ob = rlc.options(mode=rlc.MODE_ASYNC, slots=8, timing=rlc.TIMING_PER_ITERATION,
+                         group_degree=[int(x) for x in args.group_degree.split(",")], sweeps=args.sweeps, blanket=1)
+        for _ in range(args.warmup):
+            dg.run(ob, labels=False)
+        e_b, it_b = 0, []
+        t_b = time.perf_counter()
+        for _ in range(args.steps):
+            rb = dg.run(ob, labels=False)
+            e_b += rb.stats.traversed_edges
+            it_b.append(rb.iterations)
+        w_b = time.perf_counter() - t_b
+        q_b, _ = dg.modularity()
+        line["config"]["blanket_opt_in"] = {"ms_per_step": w_b * 1e3 / args.steps, "edges_per_s": e_b / w_b,
+                                            "iterations": it_b, "modularity": q_b}
+        log(f"blanket iterations (opt-in): {w_b * 1e3 / args.steps:.2f} ms per step, Q {q_b:.6g}")
     dg.close()
     ctx.close()
     if args.calibrate_copy:

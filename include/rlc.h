@@ -99,13 +99,14 @@ typedef struct rlc_options {
                              reference GPU kernels do (RESCAN=false, rakLowmemCuda.hxx:469, :624);
                              0 = rescan whenever the sketch cannot certify the choice (the CPU
                              reference's RESCAN=true result) [0]. SYNC: must be 0 */
-  int32_t always_mark;    /* ASYNC, MG8: 1 = every moved vertex flags its neighbours as it moves, as the
-                             reference does (rakMarkNeighborsCudU, rakCuda.hxx:95-103); 0 = blanket
-                             iterations (the first, and each after one whose moved vertices held at least a
-                             quarter of the edge positions) mark nothing while they run; at their end they
-                             flag every vertex if their own moved vertices hold a quarter of the edge
-                             positions, else the moved vertices mark their neighbours then [0].
-                             SYNC, BM, EXACT and batches: ignored */
+  int32_t blanket;        /* ASYNC, MG8: 0 = every moved vertex flags its neighbours as it moves, as the
+                             reference does (rakMarkNeighborsCudU, rakCuda.hxx:95-103) [0]; 1 = blanket
+                             iterations (opt-in): the first, and each after one whose moved vertices held
+                             at least rlc_run_stats::blanket_edges positions, mark nothing while they run;
+                             at their end they flag every vertex if their own moved vertices hold that many
+                             positions, else the moved vertices mark their neighbours then. Faster on large
+                             power-law graphs; it changes which vertices are processed, and on some small
+                             graphs the result (see the design notes). SYNC, BM, EXACT, batches: ignored */
   int32_t reserved[2];    /* must be zero */
 } rlc_options;
 
@@ -125,7 +126,7 @@ typedef struct rlc_run_stats {
   uint64_t changed_edges[RLC_MAX_ITER_STATS]; /* X_l: sum of degrees of changed vertices */
   float iter_ms[RLC_MAX_ITER_STATS];          /* move-launch time (RLC_TIMING_PER_ITERATION only) */
   uint64_t blanket_iterations; /* bit l: iteration l marked no neighbours and flagged every vertex at its end
-                                  (rlc_options::always_mark) */
+                                  (rlc_options::blanket) */
   uint64_t blanket_edges;      /* the fill threshold of those iterations: edge positions held by moved
                                   vertices (0 when not in use) */
 } rlc_run_stats;

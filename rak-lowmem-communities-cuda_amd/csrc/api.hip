@@ -40,13 +40,13 @@ int segments_for(uint64_t n) {
 // Everything a captured iteration loop depends on besides the graph itself.
 struct ExecKey {
   bool valid = false;
-  int sync, mg, pickstep, max_iterations, merge_degree, merge_parts, sweeps, no_rescan, always_mark;
+  int sync, mg, pickstep, max_iterations, merge_degree, merge_parts, sweeps, no_rescan, blanket;
   int group_degree[4];
   double tolerance;
   bool operator==(const ExecKey &o) const {
     return valid && o.valid && sync == o.sync && mg == o.mg && pickstep == o.pickstep &&
            max_iterations == o.max_iterations && merge_degree == o.merge_degree && merge_parts == o.merge_parts &&
-           sweeps == o.sweeps && no_rescan == o.no_rescan && always_mark == o.always_mark && tolerance == o.tolerance &&
+           sweeps == o.sweeps && no_rescan == o.no_rescan && blanket == o.blanket && tolerance == o.tolerance &&
            std::equal(group_degree, group_degree + 4, o.group_degree);
   }
 };
@@ -377,8 +377,8 @@ rlc_status rlc::check_options(const rlc_options *o) {
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.sweeps must be 0 (the graph's segments) or 1");
   if (o->timing != RLC_TIMING_GRAPH && o->timing != RLC_TIMING_PER_ITERATION)
     return fail(RLC_ERR_INVALID_ARGUMENT, "options.timing must be RLC_TIMING_GRAPH or RLC_TIMING_PER_ITERATION");
-  if (o->always_mark != 0 && o->always_mark != 1)
-    return fail(RLC_ERR_INVALID_ARGUMENT, "options.always_mark must be 0 or 1");
+  if (o->blanket != 0 && o->blanket != 1)
+    return fail(RLC_ERR_INVALID_ARGUMENT, "options.blanket must be 0 or 1");
   for (int r : o->reserved)
     if (r != 0) return fail(RLC_ERR_INVALID_ARGUMENT, "options.reserved must be zero");
   return RLC_OK;
@@ -700,7 +700,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
   const DevState ds = g->state();
   LoopParams lp{sync, o->slots == RLC_SKETCH_MG8, !o->no_rescan, o->pickstep, o->tolerance, o->max_iterations};
   lp.exact = o->slots == RLC_SKETCH_EXACT;
-  lp.blanket = !sync && !lp.exact && lp.mg && !o->always_mark;
+  lp.blanket = !sync && !lp.exact && lp.mg && o->blanket;
   if (lp.blanket) {
     if (!g->blanket_known) {
       g->blanket_edges = blanket_threshold(g->h_off);
@@ -723,7 +723,7 @@ rlc_status rlc_run(rlc_context *ctx, rlc_graph *g, const uint32_t *init_labels, 
     k.merge_parts = sync ? o->merge_parts : 0;
     k.sweeps = sync ? 0 : o->sweeps;
     k.no_rescan = o->no_rescan;
-    k.always_mark = lp.blanket ? 0 : 1;
+    k.blanket = lp.blanket ? 1 : 0;
     for (int q = 0; q < 4; ++q) k.group_degree[q] = sync ? 0 : o->group_degree[q];
     if (!(k == g->key)) {
       if (g->hexec) { (void)hipGraphExecDestroy(g->hexec); g->hexec = nullptr; }

@@ -38,7 +38,7 @@ class Options(C.Structure):
         ("repeat", C.c_int32), ("tolerance", C.c_double), ("max_iterations", C.c_int32),
         ("slots", C.c_int32), ("mode", C.c_int32), ("pickstep", C.c_int32),
         ("merge_degree", C.c_int32), ("merge_parts", C.c_int32), ("group_degree", C.c_int32 * 4),
-        ("timing", C.c_int32), ("sweeps", C.c_int32), ("no_rescan", C.c_int32), ("always_mark", C.c_int32),
+        ("timing", C.c_int32), ("sweeps", C.c_int32), ("no_rescan", C.c_int32), ("blanket", C.c_int32),
         ("reserved", C.c_int32 * 2),
     ]
 

@@ -35,7 +35,7 @@ def test_status_strings_and_defaults():
     assert L.rlc_status_string(0) == b"ok"
     o = rlc.options()
     assert (o.repeat, o.tolerance, o.max_iterations, o.slots, o.mode, o.pickstep) == (1, 0.05, 20, 8, 0, 8)
-    assert list(o.group_degree) == [0, 0, 0, 0] and list(o.reserved) == [0, 0] and o.no_rescan == 0 and o.always_mark == 0
+    assert list(o.group_degree) == [0, 0, 0, 0] and list(o.reserved) == [0, 0] and o.no_rescan == 0 and o.blanket == 0
 
 
 def test_errors_are_status_codes():

@@ -237,7 +237,8 @@ def blanket_threshold_np(g):
 @pytest.mark.parametrize("slots", [8, 1])
 def test_async_blanket_marking(ctx, slots):
     # MG8: blanket iterations mark no neighbours while they run, and at their end flag every vertex
-    # or mark the moved vertices' neighbours; always_mark = 1 (and BM) mark as the reference does
+    # or mark the moved vertices' neighbours (blanket = 1); blanket = 0 (the default) and BM mark as
+    # the reference does
     graphs = [("cfg1", cfg1()), ("rmat15", rlc.gen_rmat(15, 16, wmin=0.5, wmax=1.5, seed=2)),
               ("grid", rlc.gen_grid(120, 150, 0.1, 0.02, 3)),
               ("hubs", with_hubs(20000, hubs=40, hub_degree=9000, base_degree=10, seed=5))]
@@ -246,7 +247,7 @@ def test_async_blanket_marking(ctx, slots):
         with ctx.upload(g) as dg:
             qs = {}
             for am in (0, 1):
-                r = dg.run(rlc.options(slots=slots, always_mark=am))
+                r = dg.run(rlc.options(slots=slots, blanket=1 - am))
                 st = r.stats
                 L = r.iterations
                 bits = st.blanket_iterations
@@ -265,10 +266,10 @@ def test_async_blanket_marking(ctx, slots):
                         assert A[l] == g.n  # every vertex flagged after a filling iteration
                 qs[am], _ = dg.modularity()
                 conv = r.membership
-                print(f"{name} slots={slots} always_mark={am}: Q={qs[am]:.5f} ref={q_ref:.5f} iters={L} "
+                print(f"{name} slots={slots} reference_marks={am}: Q={qs[am]:.5f} ref={q_ref:.5f} iters={L} "
                       f"fills={bits:b} T={T} X={X} A={A}")
             # from converged labels few vertices move: the first iteration marks at its end
-            r = dg.run(rlc.options(slots=slots), init=conv)
+            r = dg.run(rlc.options(slots=slots, blanket=1), init=conv)
             st = r.stats
             X0 = int(st.changed_edges[0])
             assert X0 < st.blanket_edges and st.blanket_iterations & 1 == 0 or slots != 8
@@ -281,7 +282,7 @@ def test_async_blanket_marking(ctx, slots):
             # the deterministic mode never uses it
             assert dg.run(rlc.options(slots=slots, mode=rlc.MODE_SYNC)).stats.blanket_iterations == 0
             with pytest.raises(rlc.RlcError):
-                dg.run(rlc.options(slots=slots, always_mark=2))
+                dg.run(rlc.options(slots=slots, blanket=2))
         if slots == 8 and name in ("cfg1", "grid"):
             assert qs[0] >= q_ref * 0.99 and qs[1] >= q_ref * 0.99
         if name == "rmat15":  # no community structure (Q ~1e-4): the absolute gate
@@ -296,7 +297,7 @@ def test_async_blanket_marking_schedules(ctx, variant):
           "no-chunks": dict(group_degree=(32, 128, 512, 1 << 30))}[variant]
     for name, g in [("cfg1", cfg1()), ("hubs", with_hubs(20000, hubs=40, hub_degree=9000, base_degree=10, seed=5))]:
         with ctx.upload(g) as dg:
-            r = dg.run(rlc.options(**kw))
+            r = dg.run(rlc.options(blanket=1, **kw))
             st = r.stats
             L = r.iterations
             X = [int(st.changed_edges[l]) for l in range(L)]
@@ -307,16 +308,48 @@ def test_async_blanket_marking_schedules(ctx, variant):
                 if st.blanket_iterations >> (l - 1) & 1:
                     assert A[l] == g.n
             q, _ = dg.modularity()
-            r1 = dg.run(rlc.options(always_mark=1, **kw))
+            r1 = dg.run(rlc.options(blanket=0, **kw))
             q1, _ = dg.modularity()
             # from converged labels: the first iteration marks at its end
-            r2 = dg.run(rlc.options(**kw), init=r1.membership)
+            r2 = dg.run(rlc.options(blanket=1, **kw), init=r1.membership)
             assert r2.stats.blanket_iterations & 1 == 0
             q2, _ = dg.modularity()
-            print(f"{variant} {name}: Q={q:.5f} always_mark Q={q1:.5f} from converged Q={q2:.5f} iters={L} "
+            print(f"{variant} {name}: Q={q:.5f} reference marks Q={q1:.5f} from converged Q={q2:.5f} iters={L} "
                   f"fills={st.blanket_iterations:b}")
             if name == "cfg1":  # run-to-run spread of the asynchronous mode on these schedules: ~1%
                 assert q >= q1 - 0.02 * abs(q1) and q2 >= q1 - 0.02 * abs(q1)
+
+
+def test_async_blanket_small_graphs(ctx):
+    # flag passes on small and ragged sizes (n below and around the 16-flag vector stores, isolated
+    # vertices, no edges, self-loops): labels stay valid, disjoint cliques still reach one label
+    # each, and the fill rule holds
+    cases = [("edgeless", from_pairs(5, [])), ("one", from_pairs(1, [(0, 0)])),
+             ("star17", star(16)), ("pair+isolated", from_pairs(33, [(3, 4)])),
+             ("cliques", cliques([5, 7, 9, 4], bridges=False)), ("random", random_weighted(257, 900, seed=4))]
+    for name, g in cases:
+        with ctx.upload(g) as dg:
+            for bl in (1, 0):
+                # cliques: run to a fixpoint (no pick-less iterations, no tolerance); at a fixpoint of
+                # unit-weight label propagation a clique holds one label
+                kw = dict(tolerance=0.0, pickstep=0, max_iterations=50) if name == "cliques" else {}
+                r = dg.run(rlc.options(blanket=bl, **kw))
+                st = r.stats
+                L = r.iterations
+                lab = r.membership
+                assert lab.shape == (g.n,) and lab.max() < g.n, name
+                X = [int(st.changed_edges[l]) for l in range(L)]
+                if bl:
+                    assert st.blanket_iterations == blanket_bits_expected(X, int(st.blanket_edges), L), (name, X)
+                else:
+                    assert st.blanket_iterations == 0
+                if name == "cliques":
+                    assert st.changed_last == 0, (name, bl, L)
+                    off = g.offsets
+                    for u in range(g.n):  # every vertex ends with the label of its clique's members
+                        nb = g.edges[off[u]:off[u + 1]]
+                        assert all(lab[v] == lab[u] for v in nb if v != u), (name, bl, u)
+                print(f"{name} blanket={bl}: iters={L} fills={st.blanket_iterations:b} T={st.blanket_edges}")
 
 
 def test_async_bm_runs(ctx):
