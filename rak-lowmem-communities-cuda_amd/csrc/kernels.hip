@@ -1119,6 +1119,8 @@ void launch_mark_all(const DevGraph &g, const DevState &s, int max_iterations, b
 template <bool SYNC, bool MG, bool UNIT, bool RESCAN>
 static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &sc, const LoopParams &p,
                          hipStream_t st, const StreamFork *fk) {
+  bool hub_marks = false;  // k_chunk_mark still to launch
+  const unsigned gm = sc.chunked ? grid_for(sc.plan.nchunks, kBlock / 64, kGridCap) : 1;
   if (!SYNC && MG && sc.chunked && sc.plan.nchunks > 0) {
     const unsigned gr = grid_for(sc.plan.nchunks, kBlock / 64, kGridCap);
     const unsigned gv = grid_for(sc.plan.nslots, kBlock / 64, kGridCap);
@@ -1126,7 +1128,11 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     k_chunk_merge<<<gv, kBlock, 0, st>>>(g, s, sc.plan, RESCAN ? 1 : 0);
     if (RESCAN) k_chunk_rescan<UNIT><<<gr, kBlock, 0, st>>>(g, s, sc.plan);
     k_chunk_choose<<<gv, kBlock, 0, st>>>(g, s, sc.plan, p.pickstep);
-    k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan, 0, 0);
+    // the hubs' marks run beside the first forked sub-sweep: its vertices already read the hubs' new
+    // labels (the choice is done), and a mark that lands after one of them was processed flags it
+    // for the next iteration, as the reference's order (hubs after the low-degree kernel) does
+    if (fk) hub_marks = true;
+    else k_chunk_mark<<<gr, kBlock, 0, st>>>(g, s, sc.plan, 0, 0);
   }
   auto launch = [&](const MoveLaunch &L, hipStream_t q) {
     const uint64_t items = L.ranges.acc[L.ranges.count];
@@ -1161,12 +1167,18 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
     std::vector<size_t> ne;  // the group's non-empty launches
     for (size_t q = i; q < j; ++q)
       if (Ls[q].ranges.acc[Ls[q].ranges.count] > 0) ne.push_back(q);
-    const int k = (int)std::min<size_t>(ne.size(), StreamFork::kAux + 1);
+    const int k = (int)std::min<size_t>(ne.size(), StreamFork::kClassAux + 1);
     if (fk && k > 1) {
       // the classes of a sub-sweep touch disjoint vertices: they run together, then join (the
       // synchronous mode reads a snapshot and appends to the change list with atomics, so its
       // result does not depend on the order either)
       (void)hipEventRecord(fk->fork, st);
+      constexpr int kh = StreamFork::kAux - 1;
+      if (hub_marks) {
+        (void)hipStreamWaitEvent(fk->aux[kh], fk->fork, 0);
+        k_chunk_mark<<<gm, kBlock, 0, fk->aux[kh]>>>(g, s, sc.plan, 0, 0);
+        (void)hipEventRecord(fk->join[kh], fk->aux[kh]);
+      }
       for (int a = 1; a < k; ++a) {
         (void)hipStreamWaitEvent(fk->aux[a - 1], fk->fork, 0);
         launch(Ls[ne[a]], fk->aux[a - 1]);
@@ -1174,12 +1186,17 @@ static void launch_moves(const DevGraph &g, const DevState &s, const Schedule &s
       }
       launch(Ls[ne[0]], st);
       for (int a = 1; a < k; ++a) (void)hipStreamWaitEvent(st, fk->join[a - 1], 0);
+      if (hub_marks) {
+        (void)hipStreamWaitEvent(st, fk->join[kh], 0);
+        hub_marks = false;
+      }
       for (size_t a = (size_t)k; a < ne.size(); ++a) launch(Ls[ne[a]], st);
     } else {
       for (size_t q : ne) launch(Ls[q], st);
     }
     i = j;
   }
+  if (hub_marks) k_chunk_mark<<<gm, kBlock, 0, st>>>(g, s, sc.plan, 0, 0);  // no sub-sweep was forked
 }
 
 template <bool SYNC, bool MG>

@@ -65,7 +65,9 @@ struct MoveLaunch {
 // Streams that the degree classes of one sub-sweep run on together: the first class on the
 // caller's stream, the others on aux[], forked from and joined back to it by events.
 struct StreamFork {
-  static constexpr int kAux = 3;
+  static constexpr int kAux = 4;      // streams besides the graph's own
+  static constexpr int kClassAux = 3;  // of them, for the degree classes of a sub-sweep; the last one
+                                       // takes the chunked hubs' marks beside the first sub-sweep
   hipStream_t aux[kAux] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t join[kAux] = {};
