@@ -37,8 +37,9 @@ namespace {
 // iteration every label is distinct and few choices are certified (on R-MAT 24, vertices that
 // rescan hold 65% of the traversed edges). Later iterations certify most choices (there, ~0%),
 // and the few rescans gather the labels again, cheaper than writing 4 bytes per edge for all.
-// Either way the rescan adds the same values in the same order (synchronous mode: the snapshot
-// labels).
+// A vertex of degree at most 8 never decrements its sketch and never rescans: the lane kernel
+// stores nothing for it. Either way the rescan adds the same values in the same order
+// (synchronous mode: the snapshot labels).
 template <bool RESCAN>
 __device__ __forceinline__ bool store_labels(uint32_t iter) {
   return RESCAN && iter == 0;
@@ -83,7 +84,8 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
         MG8 sk;
         sk.clear();
         auto acc_fn = [&](uint32_t c, float w) { sk.accumulate(c, w); };
-        if (store) for_edges<UNIT, 1, decltype(acc_fn), RESCAN>(g, s.lab, u, b, e, acc_fn, s.lbuf);
+        const bool stv = store && deg > (uint64_t)kSlots;
+        if (stv) for_edges<UNIT, 1, decltype(acc_fn), RESCAN>(g, s.lab, u, b, e, acc_fn, s.lbuf);
         else for_edges<UNIT, 1>(g, s.lab, u, b, e, acc_fn);
         if (!RESCAN) {
           argmax8<!SYNC>(sk.c, sk.w, cb, wb);
@@ -94,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_move_lane(DevGraph g, DevState s, Ra
 #pragma unroll
             for (int j = 0; j < kSlots; ++j) sk.w[j] = sk.c[j] == c ? sk.w[j] + w : sk.w[j];
           };
-          if (store) for_labels<UNIT, 1>(g, s.lbuf, b, e, rs_fn);
+          if (stv) for_labels<UNIT, 1>(g, s.lbuf, b, e, rs_fn);
           else for_edges<UNIT, 1>(g, s.lab, u, b, e, rs_fn);
           argmax8<!SYNC>(sk.c, sk.w, cb, wb);
         }
