@@ -89,7 +89,8 @@ typedef struct rlc_options {
                              16 or 64 (one wavefront) lanes; below group_degree[0] one lane scans a
                              vertex; vertices of degree >= group_degree[3] are cut into 4096-edge
                              chunks, one wavefront per chunk (MG8; 256 lanes per vertex for BM).
-                             Non-decreasing; all zero = the library defaults */
+                             Non-decreasing; all zero = the library defaults (32, 128, 512, and 8192 for
+                             MG8 or 4096 for BM and the exact vote) */
   int32_t timing;         /* RLC_TIMING_* [GRAPH] */
   int32_t sweeps;         /* ASYNC: 0 = the graph's sub-sweep segments (fixed at upload: windows of 8
                              consecutive ids go round-robin into S segments, run one after the other so that
